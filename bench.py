@@ -1,0 +1,148 @@
+"""Benchmark: 3-D Yee + CPML time step on one MI355X.
+
+Workload (``voxel_3d``): 256^3 cells at 20 nm, 12-cell CPML on all six faces
+(inside the 256^3), eps per 8^3 voxel block from {1.0, 2.085, 12.1}
+(seed 20261202), Ez dipole at the centre with a 193 THz Gaussian pulse,
+FP32 fields with FP16 Cb coefficients.  One step = H update + E update
+(with CPML terms in the slabs) + source injection.
+
+Prints one JSON line: Gcell-updates/s over the timed steps, the roofline of
+the stencil (algorithmic 54 B per cell-step: 12 FP32 field accesses + 3 FP16
+coefficients, over the summed average H and E kernel durations measured with
+HIP events on the library's stream) and a CPU baseline (the NumPy FP64 O1
+oracle on the host, a few steps of the same workload, one core).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+ALG_BYTES_PER_CELL = 54.0  # 12 * 4 B fields + 3 * 2 B coefficients
+
+
+def cpu_baseline(n=256, steps=3):
+    """O1 oracle (NumPy FP64, one core) on the same workload for a few steps."""
+    import numpy as np
+
+    from oracle import compat as O0
+    from oracle.yee import YeeOracle
+    from prismo_amd.workloads import voxel_eps
+
+    pml = 12
+    d = 20e-9
+    eps = voxel_eps(n)
+    c0 = 299792458.0
+    dt = 0.99 / (c0 * np.sqrt(3 / d**2))
+    Ca, Cb, Da, Db = O0.coefficients(eps, np.ones_like(eps), np.zeros_like(eps),
+                                     np.zeros_like(eps), dt)
+    cb = {"Ex": O0._ex(Cb), "Ey": O0._ey(Cb), "Ez": O0._ez(Cb)}
+    del Ca, Da, Cb
+    o1 = YeeOracle(n, n, n, (d, d, d), dt, cb, float(Db.flat[0]), pml=(pml,) * 6)
+    o1.step()  # first step allocates the psi arrays; not timed
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        o1.step()
+        o1.F["Ez"][n // 2 - 1, n // 2 - 1, n // 2] += 1e-3
+    el = time.perf_counter() - t0
+    return {
+        "value": n**3 * steps / el / 1e9,
+        "unit": "Gcell-updates/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"NumPy FP64 textbook Yee+CPML oracle, {n}^3, {steps} steps "
+                  f"after one untimed step ({el:.1f} s), host {os.cpu_count()} cpus",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: multi-GPU runs are not part of this build "
+              "(the 3-D stencil would shard by z-slabs, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import prismo_amd as pa
+    from prismo_amd.workloads import voxel_3d
+
+    pa.set_backend("mi355x", 0)
+    sim = voxel_3d(dtype=np.float32, coefficient_storage="fp16", n=args.n)
+    eng = sim.solver.updater.engine_for(sim.fields)
+    print(f"[bench] built {args.n}^3 FP32/FP16 voxel workload", file=sys.stderr, flush=True)
+    sim.run_steps(args.warmup)
+    torch.cuda.synchronize()
+    eng.ctx.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.run_steps(args.steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    h_ms, nh = eng.ctx.timing(0)
+    e_ms, ne = eng.ctx.timing(1)
+    eng.ctx.set_timing(False)
+    cells = args.n**3
+    value = cells * args.steps / el / 1e9
+    kern_ms = h_ms + e_ms
+    achieved = ALG_BYTES_PER_CELL * cells / (kern_ms * 1e-3) / 1e9
+    finite = all(bool(torch.isfinite(sim.fields[c]).all()) for c in ("Ex", "Ey", "Ez"))
+    print(f"[bench] {args.steps} steps in {el:.3f} s, H {h_ms:.4f} ms x{nh}, E {e_ms:.4f} ms x{ne}, "
+          f"finite={finite}", file=sys.stderr, flush=True)
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("[bench] timing the CPU oracle sample", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args.n, args.cpu_steps)
+    line = {
+        "metric": "Gcell-updates/s (3-D Yee+CPML, FP32 fields / FP16 coefficients)",
+        "value": round(value, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el / args.steps * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"voxel_3d: {args.n}^3 Yee+CPML(12) at 20 nm, eps voxels 8^3 "
+                        "{1.0,2.085,12.1} seed 20261202, Ez dipole 193 THz",
+            "cells": cells,
+            "coefficients": "fp16 Cb",
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_yee_h + k_yee_e (one step)",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "alg_bytes_per_cell": ALG_BYTES_PER_CELL,
+            "kernel_ms": round(kern_ms, 5),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,203 @@
+"""ctypes binding of libpmfdtd (include/pmfdtd.h).
+
+The structures below mirror the header field for field; ``check_layout``
+compares their sizes with what the library was built with through
+``pm_abi_version``.  Loading fails loudly: there is no CPU fallback for the
+time step.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libpmfdtd.so"
+ABI_VERSION = 3
+
+PM_EX, PM_EY, PM_EZ, PM_HX, PM_HY, PM_HZ = range(6)
+COMP_ID = {"Ex": 0, "Ey": 1, "Ez": 2, "Hx": 3, "Hy": 4, "Hz": 5}
+SCHEME_COMPAT, SCHEME_YEE = 0, 1
+FP64, FP32 = 0, 1
+COEFF_FULL, COEFF_CB, COEFF_CB_F16 = 0, 1, 2
+RUN_SOURCES, RUN_SAMPLE, RUN_DFT, RUN_GRAPH = 1, 2, 4, 8
+
+ERRORS = {
+    -1: "invalid argument",
+    -2: "not supported",
+    -3: "no plan configured",
+    -4: "step outside uploaded tables",
+}
+
+vp = C.c_void_p
+i32 = C.c_int32
+i64 = C.c_int64
+f64 = C.c_double
+
+
+class pm_source(C.Structure):
+    _fields_ = [
+        ("comp", i32), ("kind", i32),
+        ("i0", i32), ("i1", i32), ("j0", i32), ("j1", i32), ("k0", i32), ("k1", i32),
+        ("table", vp), ("profile", vp), ("divisor", f64),
+    ]
+
+
+class pm_sample_box(C.Structure):
+    _fields_ = [
+        ("comp", i32),
+        ("i0", i32), ("i1", i32), ("j0", i32), ("j1", i32), ("k0", i32), ("k1", i32),
+        ("col0", i32),
+    ]
+
+
+class pm_dft_group(C.Structure):
+    _fields_ = [
+        ("col0", i32), ("col1", i32), ("f0", i32), ("f1", i32),
+        ("acc_re", vp), ("acc_im", vp), ("cmp_re", vp), ("cmp_im", vp),
+    ]
+
+
+class pm_ade_box(C.Structure):
+    _fields_ = [
+        ("comp", i32), ("kind", i32),
+        ("i0", i32), ("i1", i32), ("j0", i32), ("j1", i32), ("k0", i32), ("k1", i32),
+        ("eps_inf", f64), ("c0", f64), ("c1", f64), ("c2", f64), ("c3", f64),
+        ("cb_scale", f64), ("p", vp), ("p_prev", vp),
+    ]
+
+
+class pm_plan(C.Structure):
+    _fields_ = [
+        ("abi_version", i32), ("ndim", i32),
+        ("scheme", i32), ("precision", i32), ("coeff_kind", i32), ("pad0", i32),
+        ("nx", i32), ("ny", i32), ("nz", i32), ("pad1", i32),
+        ("sy", i64), ("sz", i64),
+        ("dx", f64), ("dy", f64), ("dz", f64), ("dt", f64),
+        ("E", vp * 3), ("H", vp * 3),
+        ("ca", vp * 3), ("cb", vp * 3), ("da", vp * 3), ("db", vp * 3),
+        ("cb_scale", f64), ("db_const", f64),
+        ("pml", i32 * 6),
+        ("pml_be", vp * 3), ("pml_ce", vp * 3), ("pml_ike", vp * 3),
+        ("pml_bh", vp * 3), ("pml_ch", vp * 3), ("pml_ikh", vp * 3),
+        ("psi_e", (vp * 2) * 3), ("psi_h", (vp * 2) * 3),
+        ("n_ade", i32), ("pad2", i32), ("ade", C.POINTER(pm_ade_box)),
+        ("n_src", i32), ("pad3", i32), ("src", C.POINTER(pm_source)),
+        ("table_step0", i64), ("table_len", i64),
+        ("n_sbox", i32), ("sample_T", i32), ("sbox", C.POINTER(pm_sample_box)),
+        ("ncols", i32), ("n_dft", i32), ("X", vp), ("dft", C.POINTER(pm_dft_group)),
+        ("W_re", vp), ("W_im", vp),
+        ("nfreq_total", i32), ("pad4", i32),
+        ("scratch", vp),
+    ]
+
+
+# Exported symbols and their signatures (the full set declared in the header).
+SIGNATURES = {
+    "pm_abi_version": ([], C.c_int),
+    "pm_create": ([C.c_int, vp, C.POINTER(vp)], C.c_int),
+    "pm_destroy": ([vp], C.c_int),
+    "pm_last_error": ([vp], C.c_char_p),
+    "pm_configure": ([vp, C.POINTER(pm_plan)], C.c_int),
+    "pm_update_h": ([vp], C.c_int),
+    "pm_update_e": ([vp], C.c_int),
+    "pm_run": ([vp, i64, i32, C.c_uint32], C.c_int),
+    "pm_dft_flush": ([vp, i64, i32], C.c_int),
+    "pm_sync": ([vp], C.c_int),
+    "pm_set_timing": ([vp, C.c_int], C.c_int),
+    "pm_get_timing": ([vp, C.c_int, C.POINTER(f64), C.POINTER(i64)], C.c_int),
+}
+
+_LIB = None
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+def load(path: os.PathLike | None = None):
+    """Load libpmfdtd once; raise if it is missing or of another ABI."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise NativeError(
+            f"{p} is missing: build it with `python -m prismo_amd.build_native` "
+            "(the mi355x time step has no CPU fallback)"
+        )
+    lib = C.CDLL(str(p))
+    for name, (args, res) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    if lib.pm_abi_version() != ABI_VERSION:
+        raise NativeError(
+            f"libpmfdtd ABI {lib.pm_abi_version()} != binding ABI {ABI_VERSION}"
+        )
+    if path is None:
+        _LIB = lib
+    return lib
+
+
+class Context:
+    """One pm_ctx on a device/stream, with status codes mapped to exceptions."""
+
+    def __init__(self, device: int = 0, stream: int | None = None):
+        self.lib = load()
+        h = vp()
+        rc = self.lib.pm_create(device, vp(stream or 0), C.byref(h))
+        if rc != 0:
+            raise NativeError(f"pm_create failed ({rc})")
+        self.h = h
+        self._keep = []
+
+    def _check(self, rc: int, what: str):
+        if rc == 0:
+            return
+        msg = self.lib.pm_last_error(self.h).decode(errors="replace")
+        kind = ERRORS.get(rc, f"HIP error {rc}")
+        if rc in (-1, -4):
+            raise ValueError(f"{what}: {kind}: {msg}")
+        if rc == -2:
+            raise NotImplementedError(f"{what}: {kind}: {msg}")
+        raise NativeError(f"{what}: {kind}: {msg}")
+
+    def configure(self, plan: pm_plan, keep=()):
+        self._keep = list(keep)  # host arrays referenced by the plan
+        self._check(self.lib.pm_configure(self.h, C.byref(plan)), "pm_configure")
+
+    def update_h(self):
+        self._check(self.lib.pm_update_h(self.h), "pm_update_h")
+
+    def update_e(self):
+        self._check(self.lib.pm_update_e(self.h), "pm_update_e")
+
+    def run(self, step0: int, nsteps: int, flags: int):
+        self._check(self.lib.pm_run(self.h, step0, nsteps, flags), "pm_run")
+
+    def dft_flush(self):
+        self._check(self.lib.pm_dft_flush(self.h, 0, 0), "pm_dft_flush")
+
+    def sync(self):
+        self._check(self.lib.pm_sync(self.h), "pm_sync")
+
+    def set_timing(self, on: bool):
+        self._check(self.lib.pm_set_timing(self.h, 1 if on else 0), "pm_set_timing")
+
+    def timing(self, which: int):
+        ms = f64()
+        n = i64()
+        self._check(self.lib.pm_get_timing(self.h, which, C.byref(ms), C.byref(n)), "pm_get_timing")
+        return ms.value, n.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.pm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

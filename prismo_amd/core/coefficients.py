@@ -1,0 +1,63 @@
+"""FP64 update coefficients and their averages at each component's points.
+
+``cell_coefficients`` is the formula of src/prismo/core/solver.py:113-133;
+``component_average`` forms the averages of solver.py:458-533 (3-D 4-point
+face averages for E, 2-point edge averages for H; 2-D variants incl. the
+identity for Hz), with the same left-to-right summation, so precomputing them
+once gives the bits the reference recomputes every step.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+EPS0 = 8.854187817e-12
+MU0 = 4 * np.pi * 1e-7
+
+
+def cell_coefficients(eps_rel, mu_rel, sigma_e, sigma_m, dt):
+    """(Ca, Cb, Da, Db) at cell nodes, FP64 (solver.py:119-130)."""
+    eps = EPS0 * eps_rel
+    sigma_dt_2eps = sigma_e * dt / (2 * eps)
+    Ca = (1 - sigma_dt_2eps) / (1 + sigma_dt_2eps)
+    Cb = (dt / eps) / (1 + sigma_dt_2eps)
+    mu = MU0 * mu_rel
+    sigma_m_dt_2mu = sigma_m * dt / (2 * mu)
+    Da = (1 - sigma_m_dt_2mu) / (1 + sigma_m_dt_2mu)
+    Db = (dt / mu) / (1 + sigma_m_dt_2mu)
+    return Ca, Cb, Da, Db
+
+
+def component_average(A: np.ndarray, comp: str, is_2d: bool) -> np.ndarray:
+    """Average of a node array at the sample points of ``comp``.
+
+    3-D: A is (nx, ny, nz); 2-D: A is (nx, ny) (the z slice of the 3-D array).
+    """
+    if is_2d:
+        if comp == "Ex" or comp == "Hy":
+            return 0.5 * (A[:, :-1] + A[:, 1:])
+        if comp == "Ey" or comp == "Hx":
+            return 0.5 * (A[:-1, :] + A[1:, :])
+        if comp == "Ez":
+            return 0.25 * (A[:-1, :-1] + A[1:, :-1] + A[:-1, 1:] + A[1:, 1:])
+        if comp == "Hz":
+            return A
+        raise ValueError(comp)
+    if comp == "Ex":
+        return 0.25 * (A[:, :-1, :-1] + A[:, 1:, :-1] + A[:, :-1, 1:] + A[:, 1:, 1:])
+    if comp == "Ey":
+        return 0.25 * (A[:-1, :, :-1] + A[1:, :, :-1] + A[:-1, :, 1:] + A[1:, :, 1:])
+    if comp == "Ez":
+        return 0.25 * (A[:-1, :-1, :] + A[1:, :-1, :] + A[:-1, 1:, :] + A[1:, 1:, :])
+    if comp == "Hx":
+        return 0.5 * (A[:-1, :, :] + A[1:, :, :])
+    if comp == "Hy":
+        return 0.5 * (A[:, :-1, :] + A[:, 1:, :])
+    if comp == "Hz":
+        return 0.5 * (A[:, :, :-1] + A[:, :, 1:])
+    raise ValueError(comp)
+
+
+def is_lossless_nonmagnetic(mu_rel, sigma_e, sigma_m) -> bool:
+    """True when Ca = Da = 1 exactly and Db is one constant."""
+    return bool(np.all(sigma_e == 0) and np.all(sigma_m == 0) and np.all(mu_rel == 1))

@@ -1,0 +1,475 @@
+// C ABI of libpmfdtd: context, plan validation, step loop, timing.
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pm_internal.h"
+
+using namespace pm;
+
+struct pm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool configured = false;
+  pm_plan plan{};
+  KGrid g{};
+  KAux aux{};
+  std::vector<KDft> dft;
+  std::string err;
+  // spectral bookkeeping: steps [pend0, pend0 + pend_n) are sampled, not yet summed
+  int64_t pend0 = 0;
+  int32_t pend_n = 0;
+  // timing
+  bool timing = false;
+  std::vector<hipEvent_t> pool;
+  size_t pool_used = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> spans[3];
+};
+
+namespace {
+
+int fail(pm_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+int hip_check(pm_ctx* c, hipError_t e, const char* what) {
+  if (e == hipSuccess) return 0;
+  return fail(c, (int)e, "%s: %s", what, hipGetErrorString(e));
+}
+
+hipEvent_t take_event(pm_ctx* c) {
+  if (c->pool_used == c->pool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    c->pool.push_back(e);
+  }
+  return c->pool[c->pool_used++];
+}
+
+struct Span {
+  pm_ctx* c;
+  int which;
+  hipEvent_t a = nullptr;
+  Span(pm_ctx* ctx, int w) : c(ctx), which(w) {
+    if (c->timing) {
+      a = take_event(c);
+      if (a) (void)hipEventRecord(a, c->stream);
+    }
+  }
+  ~Span() {
+    if (c->timing && a) {
+      hipEvent_t b = take_event(c);
+      if (b) {
+        (void)hipEventRecord(b, c->stream);
+        c->spans[which].push_back({a, b});
+      }
+    }
+  }
+};
+
+bool box_ok(int i0, int i1, int j0, int j1, int k0, int k1, const pm_plan& p) {
+  return i0 >= 0 && j0 >= 0 && k0 >= 0 && i0 <= i1 && j0 <= j1 && k0 <= k1 &&
+         i1 <= p.nx && j1 <= p.ny && k1 <= p.nz;
+}
+
+int validate(pm_ctx* c, const pm_plan& p) {
+  if (p.abi_version != PM_ABI_VERSION)
+    return fail(c, PM_EINVAL, "plan abi_version %d != library %d", p.abi_version,
+                PM_ABI_VERSION);
+  if (p.ndim != 2 && p.ndim != 3) return fail(c, PM_EINVAL, "ndim must be 2 or 3");
+  if (p.nx < 2 || p.ny < 2 || p.nz < 1) return fail(c, PM_EINVAL, "grid too small");
+  if (p.ndim == 2 && p.nz != 1) return fail(c, PM_EINVAL, "2-D plan needs nz == 1");
+  if (p.ndim == 3 && p.nz < 2) return fail(c, PM_EINVAL, "3-D plan needs nz >= 2");
+  if (p.sy < p.nx || p.sz < p.sy * p.ny)
+    return fail(c, PM_EINVAL, "strides (%lld, %lld) do not hold a %dx%d plane",
+                (long long)p.sy, (long long)p.sz, p.nx, p.ny);
+  if (!(p.dx > 0 && p.dy > 0 && p.dt > 0) || (p.ndim == 3 && !(p.dz > 0)))
+    return fail(c, PM_EINVAL, "spacings and dt must be positive");
+  if (p.precision != PM_FP64 && p.precision != PM_FP32)
+    return fail(c, PM_EINVAL, "unknown precision %d", p.precision);
+  if (p.coeff_kind < PM_COEFF_FULL || p.coeff_kind > PM_COEFF_CB_F16)
+    return fail(c, PM_EINVAL, "unknown coeff_kind %d", p.coeff_kind);
+  if (p.coeff_kind == PM_COEFF_CB_F16 && p.precision != PM_FP32)
+    return fail(c, PM_ENOTSUP, "FP16 coefficients need FP32 fields");
+
+  // which components the scheme touches
+  bool needE[3], needH[3];
+  if (p.scheme == PM_SCHEME_COMPAT) {
+    if (p.precision != PM_FP64)
+      return fail(c, PM_ENOTSUP, "compat scheme is FP64 only");
+    if (p.coeff_kind == PM_COEFF_CB_F16)
+      return fail(c, PM_ENOTSUP, "compat scheme needs FP64 coefficients");
+    for (int a = 0; a < 6; ++a)
+      if (p.pml[a]) return fail(c, PM_ENOTSUP, "compat scheme has no CPML");
+    if (p.n_ade) return fail(c, PM_ENOTSUP, "compat scheme has no ADE");
+    for (int a = 0; a < 3; ++a) needE[a] = needH[a] = true;
+    if (p.ndim == 2 && !p.scratch)
+      return fail(c, PM_EINVAL, "2-D compat needs scratch for its gates");
+  } else if (p.scheme == PM_SCHEME_YEE) {
+    for (int a = 0; a < 3; ++a) needE[a] = needH[a] = (p.ndim == 3);
+    needE[2] = needH[0] = needH[1] = true;
+  } else {
+    return fail(c, PM_EINVAL, "unknown scheme %d", p.scheme);
+  }
+  for (int a = 0; a < 3; ++a) {
+    if (needE[a] && !p.E[a]) return fail(c, PM_EINVAL, "E[%d] is NULL", a);
+    if (needH[a] && !p.H[a]) return fail(c, PM_EINVAL, "H[%d] is NULL", a);
+    if (needE[a] && !p.cb[a]) return fail(c, PM_EINVAL, "cb[%d] is NULL", a);
+    if (p.coeff_kind == PM_COEFF_FULL) {
+      if (needE[a] && !p.ca[a]) return fail(c, PM_EINVAL, "ca[%d] is NULL", a);
+      if (needH[a] && (!p.da[a] || !p.db[a]))
+        return fail(c, PM_EINVAL, "da/db[%d] is NULL", a);
+    }
+  }
+  // CPML
+  const int n[3] = {p.nx, p.ny, p.nz};
+  for (int a = 0; a < 3; ++a) {
+    const int t = p.pml[2 * a] + p.pml[2 * a + 1];
+    if (p.pml[2 * a] < 0 || p.pml[2 * a + 1] < 0 || t > n[a])
+      return fail(c, PM_EINVAL, "bad CPML thickness on axis %d", a);
+    if (t == 0) continue;
+    if (a == 2 && p.ndim == 2) return fail(c, PM_EINVAL, "2-D plan with z CPML");
+    if (!p.pml_be[a] || !p.pml_ce[a] || !p.pml_ike[a] || !p.pml_bh[a] || !p.pml_ch[a] ||
+        !p.pml_ikh[a])
+      return fail(c, PM_EINVAL, "CPML profiles missing on axis %d", a);
+    // psi[a][q]: q = 1 for component (a+2)%3 differentiated first, q = 0 the other
+    for (int q = 0; q < 2; ++q) {
+      const int comp = q == 1 ? (a + 2) % 3 : (a + 1) % 3;
+      if (needE[comp] && !p.psi_e[a][q])
+        return fail(c, PM_EINVAL, "psi_e[%d][%d] missing", a, q);
+      if (needH[comp] && !p.psi_h[a][q])
+        return fail(c, PM_EINVAL, "psi_h[%d][%d] missing", a, q);
+    }
+  }
+  if (p.n_ade < 0 || p.n_ade > kMaxAde) return fail(c, PM_EINVAL, "too many ADE boxes");
+  for (int b = 0; b < p.n_ade; ++b) {
+    const pm_ade_box& a = p.ade[b];
+    if (a.comp < 0 || a.comp > 2 || !needE[a.comp] || (a.kind != 1 && a.kind != 2))
+      return fail(c, PM_EINVAL, "ADE box %d: bad component/kind", b);
+    if (!box_ok(a.i0, a.i1, a.j0, a.j1, a.k0, a.k1, p) || !a.p || (a.kind == 1 && !a.p_prev))
+      return fail(c, PM_EINVAL, "ADE box %d: bad extent or state", b);
+  }
+  if (p.n_src < 0 || p.n_src > kMaxSrc) return fail(c, PM_EINVAL, "too many sources");
+  for (int s = 0; s < p.n_src; ++s) {
+    const pm_source& q = p.src[s];
+    if (q.comp < 0 || q.comp > 5) return fail(c, PM_EINVAL, "source %d: bad comp", s);
+    const bool present = q.comp < 3 ? needE[q.comp] : needH[q.comp - 3];
+    if (!present) return fail(c, PM_EINVAL, "source %d: component not simulated", s);
+    if (!box_ok(q.i0, q.i1, q.j0, q.j1, q.k0, q.k1, p) || !q.table ||
+        (q.kind == 1 && !q.profile) || (q.kind != 0 && q.kind != 1))
+      return fail(c, PM_EINVAL, "source %d: bad box/table", s);
+  }
+  if (p.n_src && p.table_len <= 0) return fail(c, PM_EINVAL, "empty source table");
+  if (p.n_sbox < 0 || p.n_sbox > kMaxSbox) return fail(c, PM_EINVAL, "too many monitor boxes");
+  if (p.n_sbox && (!p.X || p.sample_T <= 0 || p.ncols <= 0))
+    return fail(c, PM_EINVAL, "monitor boxes need X, sample_T and ncols");
+  for (int s = 0; s < p.n_sbox; ++s) {
+    const pm_sample_box& b = p.sbox[s];
+    if (b.comp < 0 || b.comp > 5) return fail(c, PM_EINVAL, "monitor box %d: bad comp", s);
+    const bool present = b.comp < 3 ? needE[b.comp] : needH[b.comp - 3];
+    if (!present) return fail(c, PM_EINVAL, "monitor box %d: component not simulated", s);
+    const int64_t n = (int64_t)(b.i1 - b.i0) * (b.j1 - b.j0) * (b.k1 - b.k0);
+    if (!box_ok(b.i0, b.i1, b.j0, b.j1, b.k0, b.k1, p) || b.col0 < 0 || b.col0 + n > p.ncols)
+      return fail(c, PM_EINVAL, "monitor box %d: bad extent/columns", s);
+  }
+  if (p.n_dft < 0 || p.n_dft > kMaxDft) return fail(c, PM_EINVAL, "too many DFT groups");
+  for (int s = 0; s < p.n_dft; ++s) {
+    const pm_dft_group& d = p.dft[s];
+    if (d.col0 < 0 || d.col1 > p.ncols || d.col0 >= d.col1 || d.f0 < 0 ||
+        d.f1 > p.nfreq_total || d.f0 >= d.f1 || !d.acc_re || !d.acc_im ||
+        ((d.cmp_re == nullptr) != (d.cmp_im == nullptr)))
+      return fail(c, PM_EINVAL, "DFT group %d: bad ranges/buffers", s);
+    if (!p.W_re || !p.W_im || p.table_len <= 0)
+      return fail(c, PM_EINVAL, "DFT groups need phasor tables");
+  }
+  return 0;
+}
+
+int launch_status(pm_ctx* c, const char* what) {
+  return hip_check(c, hipGetLastError(), what);
+}
+
+int do_h(pm_ctx* c) {
+  Span sp(c, 0);
+  const pm_plan& p = c->plan;
+  if (p.scheme == PM_SCHEME_COMPAT) {
+    if (p.ndim == 2) launch_compat_flags2d(c->g, c->stream);
+    launch_compat_h(c->g, p.coeff_kind, c->stream);
+  } else {
+    launch_yee_h(c->g, p.precision, p.coeff_kind, c->stream);
+  }
+  return launch_status(c, "H update launch");
+}
+
+int do_e(pm_ctx* c) {
+  Span sp(c, 1);
+  const pm_plan& p = c->plan;
+  if (p.scheme == PM_SCHEME_COMPAT)
+    launch_compat_e(c->g, p.coeff_kind, c->stream);
+  else
+    launch_yee_e(c->g, p.precision, p.coeff_kind, c->stream);
+  return launch_status(c, "E update launch");
+}
+
+int flush_pending(pm_ctx* c) {
+  const pm_plan& p = c->plan;
+  if (c->pend_n == 0) return 0;
+  const int64_t wcol0 = c->pend0 - p.table_step0;
+  if (wcol0 < 0 || wcol0 + c->pend_n > p.table_len)
+    return fail(c, PM_ERANGE, "phasor table does not cover steps %lld..%lld",
+                (long long)c->pend0, (long long)(c->pend0 + c->pend_n - 1));
+  const int esz = p.precision == PM_FP64 ? 8 : 4;
+  const char* X0 = (const char*)p.X + (c->pend0 % p.sample_T) * (int64_t)p.ncols * esz;
+  for (const KDft& d : c->dft)
+    launch_dft(d, X0, p.ncols, p.W_re, p.W_im, p.table_len, wcol0, c->pend_n, p.precision,
+               c->stream);
+  c->pend_n = 0;
+  return launch_status(c, "DFT launch");
+}
+
+}  // namespace
+
+extern "C" {
+
+int pm_abi_version(void) { return PM_ABI_VERSION; }
+
+int pm_create(int device, void* stream, pm_ctx** out) {
+  if (!out) return PM_EINVAL;
+  *out = nullptr;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess) return (int)e;
+  if (device < 0 || device >= ndev) return PM_EINVAL;
+  e = hipSetDevice(device);
+  if (e != hipSuccess) return (int)e;
+  pm_ctx* c = new pm_ctx();
+  c->device = device;
+  c->stream = (hipStream_t)stream;
+  *out = c;
+  return 0;
+}
+
+int pm_destroy(pm_ctx* c) {
+  if (!c) return 0;
+  for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
+  delete c;
+  return 0;
+}
+
+const char* pm_last_error(pm_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int pm_configure(pm_ctx* c, const pm_plan* plan) {
+  if (!c || !plan) return PM_EINVAL;
+  c->configured = false;
+  const int v = validate(c, *plan);
+  if (v) return v;
+  const pm_plan& p = *plan;
+  c->plan = p;
+  // detach the host arrays
+  c->plan.ade = nullptr;
+  c->plan.src = nullptr;
+  c->plan.sbox = nullptr;
+  c->plan.dft = nullptr;
+
+  KGrid& g = c->g;
+  g = KGrid{};
+  g.nx = p.nx;
+  g.ny = p.ny;
+  g.nz = p.nz;
+  g.ndim = p.ndim;
+  g.sy = p.sy;
+  g.sz = p.sz;
+  g.dx = p.dx;
+  g.dy = p.dy;
+  g.dz = p.dz;
+  g.dt = p.dt;
+  g.idx = 1.0 / p.dx;
+  g.idy = 1.0 / p.dy;
+  g.idz = p.ndim == 3 ? 1.0 / p.dz : 0.0;
+  for (int a = 0; a < 3; ++a) {
+    g.E[a] = p.E[a];
+    g.H[a] = p.H[a];
+    g.ca[a] = p.ca[a];
+    g.cb[a] = p.cb[a];
+    g.da[a] = p.da[a];
+    g.db[a] = p.db[a];
+    g.pml_be[a] = p.pml_be[a];
+    g.pml_ce[a] = p.pml_ce[a];
+    g.pml_ike[a] = p.pml_ike[a];
+    g.pml_bh[a] = p.pml_bh[a];
+    g.pml_ch[a] = p.pml_ch[a];
+    g.pml_ikh[a] = p.pml_ikh[a];
+    for (int q = 0; q < 2; ++q) {
+      g.psi_e[a][q] = p.psi_e[a][q];
+      g.psi_h[a][q] = p.psi_h[a][q];
+    }
+  }
+  g.cb_scale = p.cb_scale;
+  g.db_const = p.db_const;
+  g.has_pml = 0;
+  for (int a = 0; a < 6; ++a) {
+    g.pml[a] = p.pml[a];
+    if (p.pml[a]) g.has_pml = 1;
+  }
+  g.n_ade = p.n_ade;
+  for (int b = 0; b < p.n_ade; ++b) {
+    const pm_ade_box& a = p.ade[b];
+    g.ade[b] = KAde{a.comp, a.kind, a.i0, a.i1, a.j0, a.j1, a.k0, a.k1,
+                    a.eps_inf, a.c0, a.c1, a.c2, a.c3, a.cb_scale, a.p, a.p_prev};
+  }
+  g.flags = (int*)p.scratch;
+
+  KAux& x = c->aux;
+  x = KAux{};
+  x.nx = p.nx;
+  x.ny = p.ny;
+  x.nz = p.nz;
+  x.sy = p.sy;
+  x.sz = p.sz;
+  for (int a = 0; a < 3; ++a) {
+    x.F[a] = p.E[a];
+    x.F[3 + a] = p.H[a];
+  }
+  x.precision = p.precision;
+  x.n_src = p.n_src;
+  for (int s = 0; s < p.n_src; ++s) {
+    const pm_source& q = p.src[s];
+    x.src[s] = KSrc{q.comp, q.kind, q.i0, q.i1, q.j0, q.j1, q.k0, q.k1,
+                    q.table, q.profile, q.divisor};
+  }
+  x.table_step0 = p.table_step0;
+  x.table_len = p.table_len;
+  x.n_sbox = p.n_sbox;
+  x.sample_T = p.sample_T;
+  x.ncols = p.ncols;
+  x.X = p.X;
+  for (int s = 0; s < p.n_sbox; ++s) {
+    const pm_sample_box& b = p.sbox[s];
+    x.sbox[s] = KSbox{b.comp, b.i0, b.i1, b.j0, b.j1, b.k0, b.k1, b.col0};
+  }
+  c->dft.clear();
+  for (int s = 0; s < p.n_dft; ++s) {
+    const pm_dft_group& d = p.dft[s];
+    c->dft.push_back(KDft{d.col0, d.col1, d.f0, d.f1, d.acc_re, d.acc_im, d.cmp_re, d.cmp_im});
+  }
+  c->pend_n = 0;
+  c->configured = true;
+  return 0;
+}
+
+int pm_update_h(pm_ctx* c) {
+  if (!c) return PM_EINVAL;
+  if (!c->configured) return fail(c, PM_ENOPLAN, "pm_update_h before pm_configure");
+  return do_h(c);
+}
+
+int pm_update_e(pm_ctx* c) {
+  if (!c) return PM_EINVAL;
+  if (!c->configured) return fail(c, PM_ENOPLAN, "pm_update_e before pm_configure");
+  return do_e(c);
+}
+
+int pm_run(pm_ctx* c, int64_t step0, int32_t nsteps, uint32_t flags) {
+  if (!c) return PM_EINVAL;
+  if (!c->configured) return fail(c, PM_ENOPLAN, "pm_run before pm_configure");
+  if (nsteps < 0 || step0 < 0) return fail(c, PM_EINVAL, "negative step range");
+  const pm_plan& p = c->plan;
+  const bool src = (flags & PM_RUN_SOURCES) && p.n_src > 0;
+  const bool smp = (flags & PM_RUN_SAMPLE) && p.n_sbox > 0;
+  const bool dft = (flags & PM_RUN_DFT) && smp && !c->dft.empty();
+  if (flags & PM_RUN_GRAPH) return fail(c, PM_ENOTSUP, "graph replay is not built");
+  if (src || dft) {
+    const int64_t r0 = step0 - p.table_step0;
+    if (r0 < 0 || r0 + nsteps > p.table_len)
+      return fail(c, PM_ERANGE, "tables cover steps [%lld, %lld), run asks [%lld, %lld)",
+                  (long long)p.table_step0, (long long)(p.table_step0 + p.table_len),
+                  (long long)step0, (long long)(step0 + nsteps));
+  }
+  if (c->timing) {
+    c->pool_used = 0;
+    for (auto& v : c->spans) v.clear();
+  }
+  for (int32_t s = 0; s < nsteps; ++s) {
+    const int64_t n = step0 + s;
+    Span whole(c, 2);
+    int r = do_h(c);
+    if (!r) r = do_e(c);
+    if (r) return r;
+    if (src) {
+      launch_sources(c->aux, n, c->stream);
+      if ((r = launch_status(c, "source launch"))) return r;
+    }
+    if (smp) {
+      if (c->pend_n && n != c->pend0 + c->pend_n) {
+        if (dft) {
+          if ((r = flush_pending(c))) return r;
+        } else {
+          c->pend_n = 0;
+        }
+      }
+      launch_sample(c->aux, n, c->stream);
+      if ((r = launch_status(c, "sample launch"))) return r;
+      if (c->pend_n == 0) c->pend0 = n;
+      ++c->pend_n;
+      if ((n + 1) % p.sample_T == 0) {
+        if (dft) {
+          if ((r = flush_pending(c))) return r;
+        } else {
+          c->pend_n = 0;
+        }
+      }
+    }
+  }
+  return 0;
+}
+
+int pm_dft_flush(pm_ctx* c, int64_t /*step_first*/, int32_t /*rows*/) {
+  if (!c) return PM_EINVAL;
+  if (!c->configured) return fail(c, PM_ENOPLAN, "pm_dft_flush before pm_configure");
+  if (c->dft.empty()) {
+    c->pend_n = 0;
+    return 0;
+  }
+  return flush_pending(c);
+}
+
+int pm_sync(pm_ctx* c) {
+  if (!c) return PM_EINVAL;
+  return hip_check(c, hipStreamSynchronize(c->stream), "stream sync");
+}
+
+int pm_set_timing(pm_ctx* c, int enable) {
+  if (!c) return PM_EINVAL;
+  c->timing = enable != 0;
+  c->pool_used = 0;
+  for (auto& v : c->spans) v.clear();
+  return 0;
+}
+
+int pm_get_timing(pm_ctx* c, int which, double* avg_ms, int64_t* count) {
+  if (!c || which < 0 || which > 2 || !avg_ms || !count) return PM_EINVAL;
+  int r = hip_check(c, hipStreamSynchronize(c->stream), "stream sync");
+  if (r) return r;
+  double tot = 0;
+  int64_t n = 0;
+  for (auto& ab : c->spans[which]) {
+    float ms = 0;
+    hipError_t e = hipEventElapsedTime(&ms, ab.first, ab.second);
+    if (e != hipSuccess) return hip_check(c, e, "event elapsed");
+    tot += ms;
+    ++n;
+  }
+  *avg_ms = n ? tot / n : 0.0;
+  *count = n;
+  return 0;
+}
+
+}  // extern "C"

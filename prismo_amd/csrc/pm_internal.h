@@ -1,0 +1,100 @@
+// Internal types shared by the libpmfdtd translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+
+#include "../../include/pmfdtd.h"
+
+namespace pm {
+
+constexpr int kMaxAde = 8;
+constexpr int kMaxSrc = 16;
+constexpr int kMaxSbox = 24;
+constexpr int kMaxDft = 8;
+
+// Fixed block shape of the stencil kernels: one wave64 per x-row, 4 rows.
+constexpr int kBX = 64;
+constexpr int kBY = 4;
+
+struct KAde {
+  int comp, kind, i0, i1, j0, j1, k0, k1;
+  double eps_inf, c0, c1, c2, c3, cb_scale;
+  void* p;
+  void* p_prev;
+};
+
+struct KSrc {
+  int comp, kind, i0, i1, j0, j1, k0, k1;
+  const double* table;
+  const double* profile;
+  double divisor;
+};
+
+struct KSbox {
+  int comp, i0, i1, j0, j1, k0, k1, col0;
+};
+
+struct KDft {
+  int col0, col1, f0, f1;
+  void* acc_re;
+  void* acc_im;
+  void* cmp_re;
+  void* cmp_im;
+};
+
+// Everything a stencil kernel needs, passed by value.
+struct KGrid {
+  int nx, ny, nz, ndim;
+  int64_t sy, sz;
+  double dx, dy, dz, dt;
+  double idx, idy, idz;
+  void* E[3];
+  void* H[3];
+  const void* ca[3];
+  const void* cb[3];
+  const void* da[3];
+  const void* db[3];
+  double cb_scale, db_const;
+  int pml[6];
+  const void* pml_be[3];
+  const void* pml_ce[3];
+  const void* pml_ike[3];
+  const void* pml_bh[3];
+  const void* pml_ch[3];
+  const void* pml_ikh[3];
+  void* psi_e[3][2];
+  void* psi_h[3][2];
+  int has_pml;
+  int n_ade;
+  KAde ade[kMaxAde];
+  int* flags;      // compat gating flags (scratch)
+};
+
+struct KAux {
+  int nx, ny, nz;
+  int64_t sy, sz;
+  void* F[6];      // Ex Ey Ez Hx Hy Hz
+  int precision;
+  int n_src;
+  KSrc src[kMaxSrc];
+  int64_t table_step0, table_len;
+  int n_sbox, sample_T, ncols;
+  KSbox sbox[kMaxSbox];
+  void* X;
+};
+
+// Launchers (defined in the .hip files).
+void launch_yee_h(const KGrid& g, int precision, int coeff_kind, hipStream_t s);
+void launch_yee_e(const KGrid& g, int precision, int coeff_kind, hipStream_t s);
+void launch_compat_flags2d(const KGrid& g, hipStream_t s);
+void launch_compat_h(const KGrid& g, int coeff_kind, hipStream_t s);
+void launch_compat_e(const KGrid& g, int coeff_kind, hipStream_t s);
+void launch_sources(const KAux& a, int64_t step, hipStream_t s);
+void launch_sample(const KAux& a, int64_t step, hipStream_t s);
+void launch_dft(const KDft& d, const void* X, int ncols, const void* W_re,
+                const void* W_im, int64_t table_len, int64_t wcol0, int rows,
+                int precision, hipStream_t s);
+
+}  // namespace pm

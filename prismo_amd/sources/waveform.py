@@ -1,0 +1,79 @@
+"""Source time functions (src/prismo/sources/waveform.py:14-229).
+
+Each ``evaluate`` is the reference's expression in the same operation order,
+so a table built by calling it once per step holds the bits the reference
+adds to the fields.
+"""
+
+from __future__ import annotations
+
+from typing import Callable, Optional
+
+import numpy as np
+
+
+class Waveform:
+    def __init__(self, amplitude: float = 1.0, phase: float = 0.0):
+        self.amplitude = amplitude
+        self.phase = phase
+
+    def __call__(self, t):
+        return self.evaluate(t)
+
+    def evaluate(self, t):
+        raise NotImplementedError("Subclasses must implement evaluate method")
+
+    def table(self, times) -> np.ndarray:
+        """Values at each time, one scalar call per entry (bitwise as the
+        per-step calls of the reference)."""
+        return np.array([self.evaluate(float(t)) for t in times], dtype=np.float64)
+
+
+class ContinuousWave(Waveform):
+    def __init__(self, frequency: float, amplitude: float = 1.0, phase: float = 0.0):
+        super().__init__(amplitude, phase)
+        self.frequency = frequency
+        self.omega = 2 * np.pi * frequency
+
+    def evaluate(self, t):
+        return self.amplitude * np.sin(self.omega * t + self.phase)
+
+
+class GaussianPulse(Waveform):
+    """A exp(-tau^2/2) sin(w t + phi), tau = (t - delay)/width, delay 3 width."""
+
+    def __init__(self, frequency: float, pulse_width: float, amplitude: float = 1.0,
+                 phase: float = 0.0, delay: Optional[float] = None):
+        super().__init__(amplitude, phase)
+        self.frequency = frequency
+        self.pulse_width = pulse_width
+        self.omega = 2 * np.pi * frequency
+        self.delay = delay if delay is not None else 3 * pulse_width
+
+    def evaluate(self, t):
+        tau = (t - self.delay) / self.pulse_width
+        envelope = np.exp(-0.5 * tau * tau)
+        return self.amplitude * envelope * np.sin(self.omega * t + self.phase)
+
+
+class RickerWavelet(Waveform):
+    def __init__(self, frequency: float, amplitude: float = 1.0, delay: Optional[float] = None):
+        super().__init__(amplitude, 0.0)
+        self.frequency = frequency
+        self.delay = delay if delay is not None else 1.5 / frequency
+
+    def evaluate(self, t):
+        tau = np.pi * self.frequency * (t - self.delay)
+        tau2 = tau * tau
+        return self.amplitude * (1.0 - 2.0 * tau2) * np.exp(-tau2)
+
+
+class CustomWaveform(Waveform):
+    def __init__(self, waveform_func: Callable[[float], float], amplitude: float = 1.0):
+        super().__init__(amplitude, 0.0)
+        self.waveform_func = waveform_func
+
+    def evaluate(self, t):
+        if isinstance(t, np.ndarray):
+            return self.amplitude * np.array([self.waveform_func(ti) for ti in t])
+        return self.amplitude * self.waveform_func(t)

@@ -1,0 +1,69 @@
+"""The C ABI: libpmfdtd loads (no GPU needed), exports every function the
+header declares, and the ctypes mirror has the header's struct layout."""
+
+import ctypes as C
+import re
+import subprocess
+from pathlib import Path
+
+import pytest
+
+from prismo_amd import _native as N
+
+ROOT = Path(__file__).resolve().parents[1]
+HEADER = ROOT / "include" / "pmfdtd.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\*?\s+\**(pm_\w+)\s*\(", text, re.M)))
+
+
+def test_header_declares_what_binding_binds():
+    assert set(declared_functions()) == set(N.SIGNATURES)
+
+
+def test_library_loads_and_exports_all_symbols():
+    lib = N.load()
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+    assert lib.pm_abi_version() == N.ABI_VERSION
+
+
+def test_errors_without_device_are_reported():
+    lib = N.load()
+    h = C.c_void_p()
+    rc = lib.pm_create(10_000, None, C.byref(h))
+    assert rc != 0 and not h.value
+
+
+PROBE = r"""
+#include <stddef.h>
+#include <stdio.h>
+#include "pmfdtd.h"
+#define P(T, f) printf(#T "." #f " %zu\n", offsetof(T, f));
+int main(void) {
+  printf("pm_plan %zu\npm_source %zu\npm_sample_box %zu\npm_dft_group %zu\npm_ade_box %zu\n",
+         sizeof(pm_plan), sizeof(pm_source), sizeof(pm_sample_box), sizeof(pm_dft_group),
+         sizeof(pm_ade_box));
+  %FIELDS%
+  return 0;
+}
+"""
+
+
+def test_struct_layout_matches_header(tmp_path):
+    lines = []
+    for cls in (N.pm_plan, N.pm_source, N.pm_sample_box, N.pm_dft_group, N.pm_ade_box):
+        for fname, _ in cls._fields_:
+            lines.append(f"P({cls.__name__}, {fname})")
+    src = tmp_path / "probe.c"
+    src.write_text(PROBE.replace("%FIELDS%", "\n  ".join(lines)))
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-std=c11", f"-I{HEADER.parent}", str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = dict(l.rsplit(" ", 1) for l in out if l.strip())
+    for cls in (N.pm_plan, N.pm_source, N.pm_sample_box, N.pm_dft_group, N.pm_ade_box):
+        assert int(got[cls.__name__]) == C.sizeof(cls), cls.__name__
+        for fname, _ in cls._fields_:
+            assert int(got[f"{cls.__name__}.{fname}"]) == getattr(cls, fname).offset, (cls, fname)
