@@ -336,6 +336,7 @@ class StepEngine:
             sb[i].col0 = c0
         self._sbox_arr = sb
         p.n_sbox = len(boxes)
+        p.sbox = C.cast(sb, C.POINTER(N.pm_sample_box))
         p.ncols = col
         self.ncols = col
         self.sample_T = int(sample_T)

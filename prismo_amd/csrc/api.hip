@@ -393,10 +393,6 @@ int pm_run(pm_ctx* c, int64_t step0, int32_t nsteps, uint32_t flags) {
                   (long long)p.table_step0, (long long)(p.table_step0 + p.table_len),
                   (long long)step0, (long long)(step0 + nsteps));
   }
-  if (c->timing) {
-    c->pool_used = 0;
-    for (auto& v : c->spans) v.clear();
-  }
   for (int32_t s = 0; s < nsteps; ++s) {
     const int64_t n = step0 + s;
     Span whole(c, 2);
