@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define PM_ABI_VERSION 3
+#define PM_ABI_VERSION 4
 
 /* error codes (< 0) */
 #define PM_EINVAL   (-1)  /* bad argument / inconsistent plan */
@@ -178,6 +178,16 @@ typedef struct pm_plan {
 
   /* scratch owned by the caller: >= 64 bytes, 256-byte aligned */
   void* scratch;
+
+  /* Optional second buffer set for the fused step (yee, 3-D, nx <= 8 * sy
+   * / 8 with sy a multiple of 64 and sy / 64 in {2, 4, 8}).  With E2/H2 and
+   * psi_h2 (for every psi_h given) present, pm_run advances steps with one
+   * fused H+E kernel that reads one set and writes the other; a run with an
+   * odd step count does its first step in place, so every call returns with
+   * the state in E/H/psi_h.  Same layout and extents as E/H/psi_h. */
+  void* E2[3];
+  void* H2[3];
+  void* psi_h2[3][2];
 } pm_plan;
 
 typedef struct pm_ctx pm_ctx;
@@ -213,12 +223,18 @@ int pm_run(pm_ctx* ctx, int64_t step0, int32_t nsteps, uint32_t flags);
  * [step_first, step_first + rows), into every DFT group (partial block). */
 int pm_dft_flush(pm_ctx* ctx, int64_t step_first, int32_t rows);
 
+/* Launch shape of the fused step: rows per workgroup (ty in {2, 4, 8}) and
+ * z planes per workgroup (kchunk >= 1); fuse = 0 forces the unfused kernels.
+ * Defaults: fuse 1, ty 4, kchunk 16.  Used by the autotuner. */
+int pm_set_tuning(pm_ctx* ctx, int fuse, int ty, int kchunk);
+
 /* Block until all work on the context's stream has finished. */
 int pm_sync(pm_ctx* ctx);
 
 /* Average kernel time (ms) of the last pm_run, measured with HIP events on
  * the context stream around every launch of kernel class `which`
- * (0 = H update, 1 = E update, 2 = whole step); enabled by pm_set_timing. */
+ * (0 = H update, 1 = E update, 2 = whole step, 3 = fused H+E step);
+ * enabled by pm_set_timing. */
 int pm_set_timing(pm_ctx* ctx, int enable);
 int pm_get_timing(pm_ctx* ctx, int which, double* avg_ms, int64_t* count);
 

@@ -13,7 +13,7 @@ import os
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libpmfdtd.so"
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 PM_EX, PM_EY, PM_EZ, PM_HX, PM_HY, PM_HZ = range(6)
 COMP_ID = {"Ex": 0, "Ey": 1, "Ez": 2, "Hx": 3, "Hy": 4, "Hz": 5}
@@ -89,6 +89,7 @@ class pm_plan(C.Structure):
         ("W_re", vp), ("W_im", vp),
         ("nfreq_total", i32), ("pad4", i32),
         ("scratch", vp),
+        ("E2", vp * 3), ("H2", vp * 3), ("psi_h2", (vp * 2) * 3),
     ]
 
 
@@ -103,6 +104,7 @@ SIGNATURES = {
     "pm_update_e": ([vp], C.c_int),
     "pm_run": ([vp, i64, i32, C.c_uint32], C.c_int),
     "pm_dft_flush": ([vp, i64, i32], C.c_int),
+    "pm_set_tuning": ([vp, C.c_int, C.c_int, C.c_int], C.c_int),
     "pm_sync": ([vp], C.c_int),
     "pm_set_timing": ([vp, C.c_int], C.c_int),
     "pm_get_timing": ([vp, C.c_int, C.POINTER(f64), C.POINTER(i64)], C.c_int),
@@ -178,6 +180,9 @@ class Context:
 
     def dft_flush(self):
         self._check(self.lib.pm_dft_flush(self.h, 0, 0), "pm_dft_flush")
+
+    def set_tuning(self, fuse: bool = True, ty: int = 4, kchunk: int = 16):
+        self._check(self.lib.pm_set_tuning(self.h, 1 if fuse else 0, ty, kchunk), "pm_set_tuning")
 
     def sync(self):
         self._check(self.lib.pm_sync(self.h), "pm_sync")

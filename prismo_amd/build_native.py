@@ -35,6 +35,7 @@ UNITS = {
     "yee.hip": [],
     "compat.hip": ["-ffp-contract=off"],
     "aux.hip": [],
+    "fused.hip": [],
 }
 
 

@@ -50,7 +50,7 @@ def _torch():
 
 class StepEngine:
     def __init__(self, grid, dt, fields, coeffs, *, scheme: str = "compat", cpml=None,
-                 ade=(), coefficient_storage: str = "auto"):
+                 ade=(), coefficient_storage: str = "auto", fused: bool = True):
         torch = _torch()
         b = fields.backend
         if getattr(b, "name", None) != "mi355x":
@@ -86,6 +86,7 @@ class StepEngine:
         self._fill_coefficients(coeffs, coefficient_storage)
         self._fill_cpml(cpml)
         self._fill_ade(list(ade))
+        self._fill_fused(fused)
         self.plan.scratch = self._alloc((64,), torch.int32).data_ptr()
         stream = torch.cuda.current_stream(self.device)
         self.ctx = N.Context(self.device.index or 0, stream.cuda_stream)
@@ -229,6 +230,26 @@ class StepEngine:
                     p.psi_e[a][q] = self._alloc(shape, self.tdtype).data_ptr()
                 if ("Hx", "Hy", "Hz")[comp_idx] in self.need_h:
                     p.psi_h[a][q] = self._alloc(shape, self.tdtype).data_ptr()
+
+    def _fill_fused(self, fused: bool):
+        """Second buffer set for the fused ping-pong step (3-D yee, one x-row
+        per wave64: pitch / 64 in {2, 4, 8})."""
+        p = self.plan
+        self.fused = False
+        if not fused or self.scheme != "yee" or self.is_2d:
+            return
+        if self.sy % 64 or (self.sy // 64) not in (2, 4, 8):
+            return
+        nx, ny, nz = self.grid.dimensions
+        for a in range(3):
+            p.E2[a] = self._alloc((nz, ny, self.sy), self.tdtype).data_ptr()
+            p.H2[a] = self._alloc((nz, ny, self.sy), self.tdtype).data_ptr()
+        for a in range(3):
+            for q in range(2):
+                if p.psi_h[a][q]:
+                    src = next(t for t in self._keep if t.data_ptr() == p.psi_h[a][q])
+                    p.psi_h2[a][q] = self._alloc(tuple(src.shape), self.tdtype).data_ptr()
+        self.fused = True
 
     def _fill_ade(self, regions):
         p = self.plan
@@ -428,7 +449,7 @@ class StepEngine:
         p = self.plan
         for a in range(3):
             for q in range(2):
-                for v in (p.psi_e[a][q], p.psi_h[a][q]):
+                for v in (p.psi_e[a][q], p.psi_h[a][q], p.psi_h2[a][q]):
                     if v:
                         ptrs.add(v)
         return [t for t in self._keep if t.data_ptr() in ptrs]

@@ -21,11 +21,16 @@ struct pm_ctx {
   // spectral bookkeeping: steps [pend0, pend0 + pend_n) are sampled, not yet summed
   int64_t pend0 = 0;
   int32_t pend_n = 0;
+  // fused ping-pong step: parity 0 = state in plan.E/H/psi_h, 1 = in *2
+  bool fused_ok = false;
+  int fuse = 1, tune_ty = 4, tune_kchunk = 16, vec = 0;
+  int parity = 0;
+  KAux aux2{};  // source/sample view of the second buffer set
   // timing
   bool timing = false;
   std::vector<hipEvent_t> pool;
   size_t pool_used = 0;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> spans[3];
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> spans[4];
 };
 
 namespace {
@@ -219,6 +224,44 @@ int do_e(pm_ctx* c) {
   return launch_status(c, "E update launch");
 }
 
+// One fused step from buffer set `parity` into the other one.
+int do_fused(pm_ctx* c) {
+  Span sp(c, 3);
+  const pm_plan& p = c->plan;
+  FusedBufs fb{};
+  const bool a_in = c->parity == 0;
+  for (int a = 0; a < 3; ++a) {
+    fb.Ei[a] = a_in ? p.E[a] : p.E2[a];
+    fb.Hi[a] = a_in ? p.H[a] : p.H2[a];
+    fb.Eo[a] = a_in ? p.E2[a] : p.E[a];
+    fb.Ho[a] = a_in ? p.H2[a] : p.H[a];
+    for (int q = 0; q < 2; ++q) {
+      fb.psh_in[a][q] = a_in ? p.psi_h[a][q] : p.psi_h2[a][q];
+      fb.psh_out[a][q] = a_in ? p.psi_h2[a][q] : p.psi_h[a][q];
+    }
+  }
+  if (launch_yee_fused(c->g, p.precision, p.coeff_kind, fb, c->vec, c->tune_ty,
+                       c->tune_kchunk, c->stream) != 0)
+    return fail(c, PM_ENOTSUP, "fused step: no kernel for vec %d, ty %d", c->vec, c->tune_ty);
+  c->parity ^= 1;
+  return launch_status(c, "fused step launch");
+}
+
+bool fused_eligible(const pm_plan& p, int* vec) {
+  if (p.scheme != PM_SCHEME_YEE || p.ndim != 3) return false;
+  if (p.sy % 64 != 0) return false;
+  const int v = (int)(p.sy / 64);
+  if (v != 2 && v != 4 && v != 8) return false;
+  if (p.nx > 64 * v) return false;
+  for (int a = 0; a < 3; ++a) {
+    if (!p.E2[a] || !p.H2[a]) return false;
+    for (int q = 0; q < 2; ++q)
+      if (p.psi_h[a][q] && !p.psi_h2[a][q]) return false;
+  }
+  *vec = v;
+  return true;
+}
+
 int flush_pending(pm_ctx* c) {
   const pm_plan& p = c->plan;
   if (c->pend_n == 0) return 0;
@@ -360,6 +403,13 @@ int pm_configure(pm_ctx* c, const pm_plan* plan) {
     const pm_dft_group& d = p.dft[s];
     c->dft.push_back(KDft{d.col0, d.col1, d.f0, d.f1, d.acc_re, d.acc_im, d.cmp_re, d.cmp_im});
   }
+  c->aux2 = x;
+  for (int a = 0; a < 3; ++a) {
+    c->aux2.F[a] = p.E2[a];
+    c->aux2.F[3 + a] = p.H2[a];
+  }
+  c->fused_ok = fused_eligible(p, &c->vec);
+  c->parity = 0;
   c->pend_n = 0;
   c->configured = true;
   return 0;
@@ -393,14 +443,24 @@ int pm_run(pm_ctx* c, int64_t step0, int32_t nsteps, uint32_t flags) {
                   (long long)p.table_step0, (long long)(p.table_step0 + p.table_len),
                   (long long)step0, (long long)(step0 + nsteps));
   }
+  const bool fused =
+      c->fused_ok && c->fuse && fused_supported(p.precision, c->vec, c->tune_ty);
   for (int32_t s = 0; s < nsteps; ++s) {
     const int64_t n = step0 + s;
     Span whole(c, 2);
-    int r = do_h(c);
-    if (!r) r = do_e(c);
+    int r;
+    // fused runs keep an even number of ping-pong steps: an odd run does its
+    // first step in place, so the call ends with the state in the first set
+    if (fused && !(s == 0 && (nsteps & 1))) {
+      r = do_fused(c);
+    } else {
+      r = do_h(c);
+      if (!r) r = do_e(c);
+    }
     if (r) return r;
+    const KAux& ax = c->parity ? c->aux2 : c->aux;
     if (src) {
-      launch_sources(c->aux, n, c->stream);
+      launch_sources(ax, n, c->stream);
       if ((r = launch_status(c, "source launch"))) return r;
     }
     if (smp) {
@@ -411,7 +471,7 @@ int pm_run(pm_ctx* c, int64_t step0, int32_t nsteps, uint32_t flags) {
           c->pend_n = 0;
         }
       }
-      launch_sample(c->aux, n, c->stream);
+      launch_sample(ax, n, c->stream);
       if ((r = launch_status(c, "sample launch"))) return r;
       if (c->pend_n == 0) c->pend0 = n;
       ++c->pend_n;
@@ -437,6 +497,16 @@ int pm_dft_flush(pm_ctx* c, int64_t /*step_first*/, int32_t /*rows*/) {
   return flush_pending(c);
 }
 
+int pm_set_tuning(pm_ctx* c, int fuse, int ty, int kchunk) {
+  if (!c) return PM_EINVAL;
+  if (ty != 2 && ty != 4 && ty != 8) return fail(c, PM_EINVAL, "ty must be 2, 4 or 8");
+  if (kchunk < 1) return fail(c, PM_EINVAL, "kchunk must be >= 1");
+  c->fuse = fuse != 0;
+  c->tune_ty = ty;
+  c->tune_kchunk = kchunk;
+  return 0;
+}
+
 int pm_sync(pm_ctx* c) {
   if (!c) return PM_EINVAL;
   return hip_check(c, hipStreamSynchronize(c->stream), "stream sync");
@@ -451,7 +521,7 @@ int pm_set_timing(pm_ctx* c, int enable) {
 }
 
 int pm_get_timing(pm_ctx* c, int which, double* avg_ms, int64_t* count) {
-  if (!c || which < 0 || which > 2 || !avg_ms || !count) return PM_EINVAL;
+  if (!c || which < 0 || which > 3 || !avg_ms || !count) return PM_EINVAL;
   int r = hip_check(c, hipStreamSynchronize(c->stream), "stream sync");
   if (r) return r;
   double tot = 0;

@@ -85,12 +85,25 @@ struct KAux {
   void* X;
 };
 
+// One step's buffer pair for the fused kernel.
+struct FusedBufs {
+  const void* Ei[3];
+  const void* Hi[3];
+  void* Eo[3];
+  void* Ho[3];
+  const void* psh_in[3][2];
+  void* psh_out[3][2];
+};
+
 // Launchers (defined in the .hip files).
 void launch_yee_h(const KGrid& g, int precision, int coeff_kind, hipStream_t s);
 void launch_yee_e(const KGrid& g, int precision, int coeff_kind, hipStream_t s);
 void launch_compat_flags2d(const KGrid& g, hipStream_t s);
 void launch_compat_h(const KGrid& g, int coeff_kind, hipStream_t s);
 void launch_compat_e(const KGrid& g, int coeff_kind, hipStream_t s);
+int launch_yee_fused(const KGrid& g, int precision, int ck, const FusedBufs& fb, int vec,
+                     int ty, int kchunk, hipStream_t s);
+bool fused_supported(int precision, int vec, int ty);
 void launch_sources(const KAux& a, int64_t step, hipStream_t s);
 void launch_sample(const KAux& a, int64_t step, hipStream_t s);
 void launch_dft(const KDft& d, const void* X, int ncols, const void* W_re,
