@@ -33,7 +33,8 @@
 extern "C" {
 #endif
 
-#define PM_ABI_VERSION 4
+#define PM_ABI_VERSION 5
+#define PM_SCRATCH_BYTES 4096
 
 /* error codes (< 0) */
 #define PM_EINVAL   (-1)  /* bad argument / inconsistent plan */
@@ -176,11 +177,12 @@ typedef struct pm_plan {
   const void* W_im;               /*  (FP64 plain, FP32 precision) */
   int32_t nfreq_total, pad4;
 
-  /* scratch owned by the caller: >= 64 bytes, 256-byte aligned */
+  /* scratch owned by the caller: >= PM_SCRATCH_BYTES, 256-byte aligned
+   * (gating flags and the device copy of the ADE box table) */
   void* scratch;
 
-  /* Optional second buffer set for the fused step (yee, 3-D, nx <= 8 * sy
-   * / 8 with sy a multiple of 64 and sy / 64 in {2, 4, 8}).  With E2/H2 and
+  /* Optional second buffer set for the fused step (yee, 3-D, padded row
+   * sy in {128, 256, 512}).  With E2/H2 and
    * psi_h2 (for every psi_h given) present, pm_run advances steps with one
    * fused H+E kernel that reads one set and writes the other; a run with an
    * odd step count does its first step in place, so every call returns with
@@ -223,10 +225,15 @@ int pm_run(pm_ctx* ctx, int64_t step0, int32_t nsteps, uint32_t flags);
  * [step_first, step_first + rows), into every DFT group (partial block). */
 int pm_dft_flush(pm_ctx* ctx, int64_t step_first, int32_t rows);
 
-/* Launch shape of the fused step: rows per workgroup (ty in {2, 4, 8}) and
- * z planes per workgroup (kchunk >= 1); fuse = 0 forces the unfused kernels.
- * Defaults: fuse 1, ty 4, kchunk 16.  Used by the autotuner. */
-int pm_set_tuning(pm_ctx* ctx, int fuse, int ty, int kchunk);
+/* Launch shape of the fused step: rows per workgroup (ty in {2, 4}), z planes
+ * per workgroup (kchunk >= 1), cells per lane (vec in {1, 2, 4}, 0 = auto)
+ * and tile order (0: y-tiles fastest, 1: z-chunks fastest, each XCD taking a
+ * contiguous range; 2: y-tiles fastest in hardware round-robin); fuse = 0 forces
+ * the unfused kernels.  Defaults: 1, 4, 8, 0, 2.  Used by the autotuner. */
+int pm_set_tuning(pm_ctx* ctx, int fuse, int ty, int kchunk, int vec, int order);
+
+/* 1 if pm_run will use the fused kernel with the current plan and tuning. */
+int pm_fused_active(pm_ctx* ctx);
 
 /* Block until all work on the context's stream has finished. */
 int pm_sync(pm_ctx* ctx);

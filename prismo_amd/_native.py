@@ -13,7 +13,8 @@ import os
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libpmfdtd.so"
-ABI_VERSION = 4
+ABI_VERSION = 5
+SCRATCH_BYTES = 4096
 
 PM_EX, PM_EY, PM_EZ, PM_HX, PM_HY, PM_HZ = range(6)
 COMP_ID = {"Ex": 0, "Ey": 1, "Ez": 2, "Hx": 3, "Hy": 4, "Hz": 5}
@@ -104,7 +105,8 @@ SIGNATURES = {
     "pm_update_e": ([vp], C.c_int),
     "pm_run": ([vp, i64, i32, C.c_uint32], C.c_int),
     "pm_dft_flush": ([vp, i64, i32], C.c_int),
-    "pm_set_tuning": ([vp, C.c_int, C.c_int, C.c_int], C.c_int),
+    "pm_set_tuning": ([vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
+    "pm_fused_active": ([vp], C.c_int),
     "pm_sync": ([vp], C.c_int),
     "pm_set_timing": ([vp, C.c_int], C.c_int),
     "pm_get_timing": ([vp, C.c_int, C.POINTER(f64), C.POINTER(i64)], C.c_int),
@@ -122,7 +124,7 @@ def load(path: os.PathLike | None = None):
     global _LIB
     if _LIB is not None and path is None:
         return _LIB
-    p = Path(path) if path else LIB_PATH
+    p = Path(path) if path else Path(os.environ.get("PRISMO_AMD_LIB", LIB_PATH))
     if not p.exists():
         raise NativeError(
             f"{p} is missing: build it with `python -m prismo_amd.build_native` "
@@ -181,8 +183,13 @@ class Context:
     def dft_flush(self):
         self._check(self.lib.pm_dft_flush(self.h, 0, 0), "pm_dft_flush")
 
-    def set_tuning(self, fuse: bool = True, ty: int = 4, kchunk: int = 16):
-        self._check(self.lib.pm_set_tuning(self.h, 1 if fuse else 0, ty, kchunk), "pm_set_tuning")
+    def set_tuning(self, fuse: bool = True, ty: int = 4, kchunk: int = 16, vec: int = 0,
+                   order: int = 0):
+        self._check(self.lib.pm_set_tuning(self.h, 1 if fuse else 0, ty, kchunk, vec, order),
+                    "pm_set_tuning")
+
+    def fused_active(self) -> bool:
+        return bool(self.lib.pm_fused_active(self.h))
 
     def sync(self):
         self._check(self.lib.pm_sync(self.h), "pm_sync")

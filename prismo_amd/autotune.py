@@ -1,0 +1,107 @@
+"""Autotuner for the fused Yee step's launch shape.
+
+Times each candidate (cells per lane, rows per workgroup, z planes per
+workgroup, tile order) for a few steps with HIP events on the library's own
+stream, keeps the fastest, and restores the simulation state it started from:
+fields, CPML memory and dispersive state are snapshotted on the device first,
+so tuning leaves a simulation exactly where it was.  Results are cached per
+(grid, precision, coefficient storage, CPML, ADE) in memory and, when
+``PRISMO_AMD_TUNE_CACHE`` names a file, in that JSON file.
+"""
+
+from __future__ import annotations
+
+import json
+import os
+from itertools import product
+
+_CACHE: dict = {}
+
+CANDIDATES = {
+    "vec": (1, 2),
+    "ty": (2, 4),
+    "kchunk": (4, 8, 16),
+    "order": (0, 2),
+}
+
+
+def _key(eng) -> str:
+    p = eng.plan
+    return json.dumps([p.nx, p.ny, p.nz, int(p.precision), int(p.coeff_kind),
+                       list(p.pml), int(p.n_ade)])
+
+
+def _load_file():
+    path = os.environ.get("PRISMO_AMD_TUNE_CACHE")
+    if path and os.path.exists(path):
+        try:
+            _CACHE.update(json.load(open(path)))
+        except (OSError, ValueError):
+            pass
+
+
+def _save_file():
+    path = os.environ.get("PRISMO_AMD_TUNE_CACHE")
+    if path:
+        try:
+            json.dump(_CACHE, open(path, "w"))
+        except OSError:
+            pass
+
+
+def _state_tensors(eng):
+    """Every device tensor a step reads and writes in place."""
+    out = [eng.fields[c] for c in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")]
+    out += eng._psi_tensors()
+    for P, Pm in getattr(eng, "ade_state", []):
+        out.append(P)
+        if Pm is not None:
+            out.append(Pm)
+    return out
+
+
+def tune(eng, steps: int = 6, warmup: int = 2, candidates: dict | None = None,
+         force: bool = False) -> dict:
+    """Pick and apply the fastest fused-step launch shape for ``eng``.
+
+    Returns {"vec", "ty", "kchunk", "order", "ms"} (``fused`` False when the
+    plan cannot take the fused step; then nothing is changed)."""
+    import torch
+
+    if not getattr(eng, "fused", False):
+        return {"fused": False}
+    _load_file()
+    key = _key(eng)
+    if key in _CACHE and not force:
+        best = _CACHE[key]
+        eng.ctx.set_tuning(True, best["ty"], best["kchunk"], best["vec"], best["order"])
+        return dict(best, fused=True, cached=True)
+    cand = candidates or CANDIDATES
+    saved = [t.clone() for t in _state_tensors(eng)]
+    best = None
+    try:
+        for vec, ty, kc, order in product(cand["vec"], cand["ty"], cand["kchunk"], cand["order"]):
+            try:
+                eng.ctx.set_tuning(True, ty, kc, vec, order)
+            except ValueError:
+                continue
+            if not eng.ctx.fused_active():
+                continue
+            eng.ctx.run(0, warmup if warmup % 2 == 0 else warmup + 1, 0)
+            eng.ctx.set_timing(True)
+            eng.ctx.run(0, steps if steps % 2 == 0 else steps + 1, 0)
+            ms, n = eng.ctx.timing(3)
+            eng.ctx.set_timing(False)
+            if n and (best is None or ms < best["ms"]):
+                best = {"vec": vec, "ty": ty, "kchunk": kc, "order": order, "ms": ms}
+    finally:
+        for t, s in zip(_state_tensors(eng), saved):
+            t.copy_(s)
+        torch.cuda.synchronize(eng.device)
+    if best is None:
+        eng.ctx.set_tuning(True, 4, 8, 0, 2)
+        return {"fused": False}
+    eng.ctx.set_tuning(True, best["ty"], best["kchunk"], best["vec"], best["order"])
+    _CACHE[key] = best
+    _save_file()
+    return dict(best, fused=True, cached=False)

@@ -35,7 +35,8 @@ UNITS = {
     "yee.hip": [],
     "compat.hip": ["-ffp-contract=off"],
     "aux.hip": [],
-    "fused.hip": [],
+    "fused_f32.hip": [],
+    "fused_f64.hip": [],
 }
 
 

@@ -87,7 +87,7 @@ class StepEngine:
         self._fill_cpml(cpml)
         self._fill_ade(list(ade))
         self._fill_fused(fused)
-        self.plan.scratch = self._alloc((64,), torch.int32).data_ptr()
+        self.plan.scratch = self._alloc((N.SCRATCH_BYTES // 4,), torch.int32).data_ptr()
         stream = torch.cuda.current_stream(self.device)
         self.ctx = N.Context(self.device.index or 0, stream.cuda_stream)
         self._io_keep: list = []
@@ -238,7 +238,7 @@ class StepEngine:
         self.fused = False
         if not fused or self.scheme != "yee" or self.is_2d:
             return
-        if self.sy % 64 or (self.sy // 64) not in (2, 4, 8):
+        if self.sy not in (128, 256, 512):
             return
         nx, ny, nz = self.grid.dimensions
         for a in range(3):

@@ -104,6 +104,13 @@ class Simulation:
             rng.append((0, 1))
         return tuple(rng)
 
+    def autotune(self, force: bool = False) -> dict:
+        """Pick the fastest launch shape of the fused step for this grid
+        (state is restored afterwards; see ``prismo_amd.autotune``)."""
+        from ..autotune import tune
+
+        return tune(self.solver.updater.engine_for(self.fields), force=force)
+
     # -- running --------------------------------------------------------------------
     def run(self, time: float, progress_callback=None, progress_interval: int = 10) -> None:
         steps = int(np.ceil(time / self.dt))

@@ -1,6 +1,7 @@
 // C ABI of libpmfdtd: context, plan validation, step loop, timing.
 #include <cstdarg>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -23,9 +24,10 @@ struct pm_ctx {
   int32_t pend_n = 0;
   // fused ping-pong step: parity 0 = state in plan.E/H/psi_h, 1 = in *2
   bool fused_ok = false;
-  int fuse = 1, tune_ty = 4, tune_kchunk = 16, vec = 0;
+  int fuse = 1, tune_ty = 4, tune_kchunk = 8, tune_vec = 0, tune_order = 2;
   int parity = 0;
   KAux aux2{};  // source/sample view of the second buffer set
+  std::vector<KAde> ade_host;  // staging of the ADE table upload
   // timing
   bool timing = false;
   std::vector<hipEvent_t> pool;
@@ -155,6 +157,8 @@ int validate(pm_ctx* c, const pm_plan& p) {
     }
   }
   if (p.n_ade < 0 || p.n_ade > kMaxAde) return fail(c, PM_EINVAL, "too many ADE boxes");
+  if (!p.scratch) return fail(c, PM_EINVAL, "scratch (PM_SCRATCH_BYTES) is required");
+  static_assert(kScratchAde + sizeof(KAde) * kMaxAde <= PM_SCRATCH_BYTES, "scratch layout");
   for (int b = 0; b < p.n_ade; ++b) {
     const pm_ade_box& a = p.ade[b];
     if (a.comp < 0 || a.comp > 2 || !needE[a.comp] || (a.kind != 1 && a.kind != 2))
@@ -224,6 +228,15 @@ int do_e(pm_ctx* c) {
   return launch_status(c, "E update launch");
 }
 
+// Cells per lane and waves per row of the fused kernel for this plan.
+void fused_shape(const pm_ctx* c, int* vec, int* w) {
+  const int rw = (int)c->plan.sy;  // padded row, 128 / 256 / 512
+  int v = c->tune_vec;
+  if (v <= 0) v = c->plan.precision == PM_FP32 ? (rw >= 128 ? 2 : 1) : (rw <= 256 ? 1 : 2);
+  *vec = v;
+  *w = rw / (64 * v);
+}
+
 // One fused step from buffer set `parity` into the other one.
 int do_fused(pm_ctx* c) {
   Span sp(c, 3);
@@ -240,25 +253,32 @@ int do_fused(pm_ctx* c) {
       fb.psh_out[a][q] = a_in ? p.psi_h2[a][q] : p.psi_h[a][q];
     }
   }
-  if (launch_yee_fused(c->g, p.precision, p.coeff_kind, fb, c->vec, c->tune_ty,
-                       c->tune_kchunk, c->stream) != 0)
-    return fail(c, PM_ENOTSUP, "fused step: no kernel for vec %d, ty %d", c->vec, c->tune_ty);
+  int vec, w;
+  fused_shape(c, &vec, &w);
+  const int r = p.precision == PM_FP32
+                    ? launch_yee_fused_f32(c->g, p.coeff_kind, fb, vec, w, c->tune_ty,
+                                           c->tune_kchunk, c->tune_order, c->stream)
+                    : launch_yee_fused_f64(c->g, p.coeff_kind, fb, vec, w, c->tune_ty,
+                                           c->tune_kchunk, c->tune_order, c->stream);
+  if (r != 0)
+    return fail(c, PM_ENOTSUP, "fused step: no kernel for vec %d, w %d, ty %d", vec, w,
+                c->tune_ty);
   c->parity ^= 1;
   return launch_status(c, "fused step launch");
 }
 
-bool fused_eligible(const pm_plan& p, int* vec) {
+bool fused_eligible(const pm_plan& p) {
   if (p.scheme != PM_SCHEME_YEE || p.ndim != 3) return false;
-  if (p.sy % 64 != 0) return false;
-  const int v = (int)(p.sy / 64);
-  if (v != 2 && v != 4 && v != 8) return false;
-  if (p.nx > 64 * v) return false;
+  if (p.sy != 128 && p.sy != 256 && p.sy != 512) return false;
+  if (p.nx > p.sy) return false;
+  // the fused kernel addresses fields with 32-bit byte offsets
+  const uint64_t esz = p.precision == PM_FP64 ? 8 : 4;
+  if ((uint64_t)p.sz * (uint64_t)p.nz * esz >= (1ull << 32)) return false;
   for (int a = 0; a < 3; ++a) {
     if (!p.E2[a] || !p.H2[a]) return false;
     for (int q = 0; q < 2; ++q)
       if (p.psi_h[a][q] && !p.psi_h2[a][q]) return false;
   }
-  *vec = v;
   return true;
 }
 
@@ -363,12 +383,30 @@ int pm_configure(pm_ctx* c, const pm_plan* plan) {
     if (p.pml[a]) g.has_pml = 1;
   }
   g.n_ade = p.n_ade;
+  g.flags = (int*)((char*)p.scratch + kScratchFlags);
+  g.ade = (const KAde*)((char*)p.scratch + kScratchAde);
+  g.ade_j0 = g.ade_k0 = 1 << 30;
+  g.ade_j1 = g.ade_k1 = -1;
   for (int b = 0; b < p.n_ade; ++b) {
-    const pm_ade_box& a = p.ade[b];
-    g.ade[b] = KAde{a.comp, a.kind, a.i0, a.i1, a.j0, a.j1, a.k0, a.k1,
-                    a.eps_inf, a.c0, a.c1, a.c2, a.c3, a.cb_scale, a.p, a.p_prev};
+    g.ade_j0 = std::min(g.ade_j0, p.ade[b].j0);
+    g.ade_j1 = std::max(g.ade_j1, p.ade[b].j1);
+    g.ade_k0 = std::min(g.ade_k0, p.ade[b].k0);
+    g.ade_k1 = std::max(g.ade_k1, p.ade[b].k1);
   }
-  g.flags = (int*)p.scratch;
+  if (p.n_ade) {
+    KAde tab[kMaxAde];
+    for (int b = 0; b < p.n_ade; ++b) {
+      const pm_ade_box& a = p.ade[b];
+      tab[b] = KAde{a.comp, a.kind, a.i0, a.i1, a.j0, a.j1, a.k0, a.k1,
+                    a.eps_inf, a.c0, a.c1, a.c2, a.c3, a.cb_scale, a.p, a.p_prev};
+    }
+    // stream-ordered upload into caller memory; the host copy lives in ctx
+    c->ade_host.assign(tab, tab + p.n_ade);
+    int r = hip_check(c, hipMemcpyAsync((void*)g.ade, c->ade_host.data(),
+                                        sizeof(KAde) * p.n_ade, hipMemcpyHostToDevice,
+                                        c->stream), "ADE table upload");
+    if (r) return r;
+  }
 
   KAux& x = c->aux;
   x = KAux{};
@@ -408,7 +446,7 @@ int pm_configure(pm_ctx* c, const pm_plan* plan) {
     c->aux2.F[a] = p.E2[a];
     c->aux2.F[3 + a] = p.H2[a];
   }
-  c->fused_ok = fused_eligible(p, &c->vec);
+  c->fused_ok = fused_eligible(p);
   c->parity = 0;
   c->pend_n = 0;
   c->configured = true;
@@ -443,8 +481,10 @@ int pm_run(pm_ctx* c, int64_t step0, int32_t nsteps, uint32_t flags) {
                   (long long)p.table_step0, (long long)(p.table_step0 + p.table_len),
                   (long long)step0, (long long)(step0 + nsteps));
   }
+  int fvec = 0, fw = 0;
+  if (c->fused_ok) fused_shape(c, &fvec, &fw);
   const bool fused =
-      c->fused_ok && c->fuse && fused_supported(p.precision, c->vec, c->tune_ty);
+      c->fused_ok && c->fuse && fused_supported(p.precision, fvec, fw, c->tune_ty);
   for (int32_t s = 0; s < nsteps; ++s) {
     const int64_t n = step0 + s;
     Span whole(c, 2);
@@ -497,14 +537,27 @@ int pm_dft_flush(pm_ctx* c, int64_t /*step_first*/, int32_t /*rows*/) {
   return flush_pending(c);
 }
 
-int pm_set_tuning(pm_ctx* c, int fuse, int ty, int kchunk) {
+int pm_set_tuning(pm_ctx* c, int fuse, int ty, int kchunk, int vec, int order) {
   if (!c) return PM_EINVAL;
-  if (ty != 2 && ty != 4 && ty != 8) return fail(c, PM_EINVAL, "ty must be 2, 4 or 8");
+  if (ty != 2 && ty != 4) return fail(c, PM_EINVAL, "ty must be 2 or 4");
   if (kchunk < 1) return fail(c, PM_EINVAL, "kchunk must be >= 1");
+  if (vec != 0 && vec != 1 && vec != 2 && vec != 4)
+    return fail(c, PM_EINVAL, "vec must be 0 (auto), 1, 2 or 4");
+  if (order < 0 || order > 2) return fail(c, PM_EINVAL, "order must be 0, 1 or 2");
   c->fuse = fuse != 0;
   c->tune_ty = ty;
   c->tune_kchunk = kchunk;
+  c->tune_vec = vec;
+  c->tune_order = order;
   return 0;
+}
+
+int pm_fused_active(pm_ctx* c) {
+  if (!c || !c->configured) return 0;
+  int vec = 0, w = 0;
+  if (!c->fused_ok) return 0;
+  fused_shape(c, &vec, &w);
+  return (c->fuse && fused_supported(c->plan.precision, vec, w, c->tune_ty)) ? 1 : 0;
 }
 
 int pm_sync(pm_ctx* c) {

@@ -68,9 +68,14 @@ struct KGrid {
   void* psi_h[3][2];
   int has_pml;
   int n_ade;
-  KAde ade[kMaxAde];
-  int* flags;      // compat gating flags (scratch)
+  int ade_j0, ade_j1, ade_k0, ade_k1;  // union of the boxes' j and k ranges
+  const KAde* ade;  // device copy of the ADE boxes (in scratch)
+  int* flags;       // compat gating flags (scratch)
 };
+
+// scratch layout (caller-owned, PM_SCRATCH_BYTES)
+constexpr int kScratchFlags = 0;
+constexpr int kScratchAde = 1024;
 
 struct KAux {
   int nx, ny, nz;
@@ -101,9 +106,11 @@ void launch_yee_e(const KGrid& g, int precision, int coeff_kind, hipStream_t s);
 void launch_compat_flags2d(const KGrid& g, hipStream_t s);
 void launch_compat_h(const KGrid& g, int coeff_kind, hipStream_t s);
 void launch_compat_e(const KGrid& g, int coeff_kind, hipStream_t s);
-int launch_yee_fused(const KGrid& g, int precision, int ck, const FusedBufs& fb, int vec,
-                     int ty, int kchunk, hipStream_t s);
-bool fused_supported(int precision, int vec, int ty);
+int launch_yee_fused_f32(const KGrid& g, int ck, const FusedBufs& fb, int vec, int w, int ty,
+                         int kchunk, int order, hipStream_t s);
+int launch_yee_fused_f64(const KGrid& g, int ck, const FusedBufs& fb, int vec, int w, int ty,
+                         int kchunk, int order, hipStream_t s);
+bool fused_supported(int precision, int vec, int w, int ty);
 void launch_sources(const KAux& a, int64_t step, hipStream_t s);
 void launch_sample(const KAux& a, int64_t step, hipStream_t s);
 void launch_dft(const KDft& d, const void* X, int ncols, const void* W_re,

@@ -71,6 +71,9 @@ __device__ __forceinline__ T load_cb(const KGrid& g, int c, int64_t off) {
 template <typename T>
 __device__ __forceinline__ bool ade_update(const KGrid& g, int comp, int i, int j,
                                            int k, T curl, T& e) {
+  // the box table lives in device memory, so a rolled loop indexes it
+  // without copying kernel arguments to private memory
+#pragma unroll 1
   for (int b = 0; b < g.n_ade; ++b) {
     const KAde& a = g.ade[b];
     if (a.comp != comp || i < a.i0 || i >= a.i1 || j < a.j0 || j >= a.j1 ||
