@@ -7,10 +7,22 @@ FP32 fields with FP16 Cb coefficients.  One step = H update + E update
 (with CPML terms in the slabs) + source injection.
 
 Prints one JSON line: Gcell-updates/s over the timed steps, the roofline of
-the stencil (algorithmic 54 B per cell-step: 12 FP32 field accesses + 3 FP16
-coefficients, over the summed average H and E kernel durations measured with
-HIP events on the library's stream) and a CPU baseline (the NumPy FP64 O1
-oracle on the host, a few steps of the same workload, one core).
+the fused step kernel and a CPU baseline (the NumPy FP64 O1 oracle on the
+host, a few steps of the same workload, one core).
+
+Algorithmic bytes of one step (no reuse counted): 54 B per cell (6 FP32
+components read and written, 3 FP16 Cb coefficients) plus 32 B per
+(cell, CPML axis) inside the slabs (two E-side and two H-side FP32 psi
+values read and written).  ``achieved`` divides that by the average duration
+of the fused kernel, timed with HIP events on the library's own stream over
+the timed steps.  ``traffic`` is the HBM bytes one launch moves as measured
+by the rocprofv3 FETCH_SIZE / WRITE_SIZE counters (scaled by a copy-kernel
+calibration), read from the committed profile summary for this workload
+when it is present.
+
+``--copy-probe`` runs only a 256 MiB device copy (the counter calibration
+kernel); ``--launch vec,ty,kchunk,order`` fixes the fused launch shape
+instead of autotuning.
 """
 
 from __future__ import annotations
@@ -26,6 +38,52 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 ALG_BYTES_PER_CELL = 54.0  # 12 * 4 B fields + 3 * 2 B coefficients
+ALG_BYTES_PER_PML_CELL_AXIS = 32.0  # 4 psi * 4 B, read and written
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_yee_fused_voxel256_traffic.md")
+
+
+def algorithmic_bytes(plan) -> float:
+    n = (plan.nx, plan.ny, plan.nz)
+    cells = n[0] * n[1] * n[2]
+    slab = 0
+    for a in range(3):
+        slab += (plan.pml[2 * a] + plan.pml[2 * a + 1]) * cells // n[a]
+    return ALG_BYTES_PER_CELL * cells + ALG_BYTES_PER_PML_CELL_AXIS * slab
+
+
+def measured_traffic(workload_key: str, launch: dict):
+    """HBM bytes per fused launch from the committed counter summary, when
+    it covers this workload and launch shape (else None)."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    vals = {}
+    for line in open(TRAFFIC_FILE):
+        if line.startswith("traffic:"):
+            for kv in line[len("traffic:"):].split():
+                k, _, v = kv.partition("=")
+                vals[k] = v
+    try:
+        if vals.get("workload") != workload_key:
+            return None
+        shape = tuple(int(x) for x in vals["launch"].split(","))
+        if shape != tuple(launch[k] for k in ("vec", "ty", "kchunk", "order")):
+            return None
+        return float(vals["bytes"])
+    except (KeyError, ValueError, TypeError):
+        return None
+
+
+def copy_probe():
+    """256 MiB device-to-device copy, four times: the kernel whose counter
+    readings calibrate FETCH_SIZE and WRITE_SIZE."""
+    import torch
+
+    a = torch.ones(256 << 18, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(4):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"[bench] copy probe: 4 x {a.numel() * 4} B copied", file=sys.stderr, flush=True)
 
 
 def cpu_baseline(n=256, steps=3):
@@ -69,12 +127,17 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--launch", type=str, default=None)
+    ap.add_argument("--copy-probe", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: multi-GPU runs are not part of this build "
-              "(the 3-D stencil would shard by z-slabs, see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only; runs on several GPUs are not supported "
+              "(the 3-D stencil would shard by z-slabs)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe()
+        return
 
     import numpy as np
     import torch
@@ -86,8 +149,13 @@ def main():
     sim = voxel_3d(dtype=np.float32, coefficient_storage="fp16", n=args.n)
     eng = sim.solver.updater.engine_for(sim.fields)
     print(f"[bench] built {args.n}^3 FP32/FP16 voxel workload", file=sys.stderr, flush=True)
-    tuned = sim.autotune()
-    print(f"[bench] autotuned fused step: {tuned}", file=sys.stderr, flush=True)
+    if args.launch:
+        vec, ty, kc, order = (int(x) for x in args.launch.split(","))
+        eng.ctx.set_tuning(True, ty, kc, vec, order)
+        tuned = {"vec": vec, "ty": ty, "kchunk": kc, "order": order, "fused": True}
+    else:
+        tuned = sim.autotune()
+    print(f"[bench] fused step launch: {tuned}", file=sys.stderr, flush=True)
     sim.run_steps(args.warmup)
     torch.cuda.synchronize()
     eng.ctx.set_timing(True)
@@ -109,7 +177,11 @@ def main():
     else:
         kern_ms, kname = h_ms + e_ms, "k_yee_h + k_yee_e (one step)"
 
-    achieved = ALG_BYTES_PER_CELL * cells / (kern_ms * 1e-3) / 1e9
+    alg_bytes = algorithmic_bytes(eng.plan)
+    achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
+    launch = {k: tuned.get(k) for k in ("vec", "ty", "kchunk", "order")}
+    wkey = f"voxel_3d_{args.n}_fp32_fp16"
+    traffic = measured_traffic(wkey, launch) if nf >= nh else None
     finite = all(bool(torch.isfinite(sim.fields[c]).all()) for c in ("Ex", "Ey", "Ez"))
     print(f"[bench] {args.steps} steps in {el:.3f} s, fused {f_ms:.4f} ms x{nf}, "
           f"H {h_ms:.4f} ms x{nh}, E {e_ms:.4f} ms x{ne}, "
@@ -137,7 +209,7 @@ def main():
             "cells": cells,
             "coefficients": "fp16 Cb",
             "parallelism": "single GPU",
-            "launch": {k: tuned.get(k) for k in ("vec", "ty", "kchunk", "order")},
+            "launch": launch,
         },
         "roofline": {
             "bound": "hbm",
@@ -146,8 +218,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "alg_bytes_per_cell": ALG_BYTES_PER_CELL,
+            "traffic": traffic,
+            "alg_bytes_per_launch": alg_bytes,
             "kernel_ms": round(kern_ms, 5),
         },
         "cpu_baseline": cpu,

@@ -103,18 +103,24 @@ typedef struct pm_dft_group {
 } pm_dft_group;
 
 /* Dispersive (ADE) region for one E component: a box of cells with its own
- * state arrays in box order.  Coefficients follow
- * src/prismo/materials/dispersion.py:189-231 (Lorentz) and 267-286 (Drude)
- * in units where P and J are divided by eps0. */
+ * state arrays in box order (zero-filled by the caller before the first
+ * step).  c0..c3 are the coefficients exactly as
+ * src/prismo/materials/dispersion.py:189-231 (Lorentz, one pole) and
+ * 267-286 (Drude) return them; P and J are in units of eps0.  The library
+ * couples them to the E update with the drive averaged over the half steps
+ * on either side (time-centred), so the discrete eps(w) matches the closed
+ * form to O((w dt)^2):
+ *   Lorentz: P+ = c2 P + c3 P- + [(c0 E+ + c1 E) + (c0 E + c1 E-)] / 2
+ *   Drude:   J+ = c1 J + c0 (E+ + E) / 2
+ *   eps_inf (E+ - E) + (P+ - P) = cb_scale curlH,  P+ - P = dt (J+ + J) / 2. */
 typedef struct pm_ade_box {
   int32_t comp, kind;            /* comp in 0..2; kind 1 Lorentz, 2 Drude */
   int32_t i0, i1, j0, j1, k0, k1;
-  double eps_inf;
-  double c0, c1, c2, c3;          /* Lorentz: P+ = c0 E+ + c1 E + c2 P + c3 P- ;
-                                     Drude:   J+ = c0 E+ + c1 J            */
-  double cb_scale;                /* dt / eps0 (curl H -> E units) */
+  double eps_inf;                /* > 0 */
+  double c0, c1, c2, c3;          /* Lorentz C0..C3; Drude C0, C1 (c2, c3 unused) */
+  double cb_scale;                /* dt / eps0 (curl H -> E units), > 0 */
   void* p;                        /* P (Lorentz) or J (Drude) */
-  void* p_prev;                   /* P- (Lorentz only) */
+  void* p_prev;                   /* Lorentz: second state, Q = c3 P- + c1 E- / 2 */
 } pm_ade_box;
 
 typedef struct pm_plan {

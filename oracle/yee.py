@@ -14,9 +14,12 @@ Scheme (stated here, pinned by the physics tests in tests/test_oracle_*):
   * CPML (Roden & Gedney 2000), full-size psi arrays, b/c/kappa profiles from
     ``cpml_profiles`` below (an independent restatement of the textbook
     grading, compared with the product's in the tests);
-  * ADE: Lorentz P+ = c0 E+ + c1 E + c2 P + c3 P-, Drude J+ = c0 E+ + c1 J
-    (coefficients of src/prismo/materials/dispersion.py:189-231, 267-286),
-    coupled through D = eps_inf E + P (units of eps0);
+  * ADE with the coefficients of src/prismo/materials/dispersion.py:189-231
+    (Lorentz C0..C3) and 267-286 (Drude C0, C1), coupled through
+    D = eps_inf E + P (units of eps0) with the drive averaged over the two
+    half steps around each update (time-centred):
+      Lorentz  P+ = C2 P + C3 P- + [(C0 E+ + C1 E) + (C0 E + C1 E-)] / 2
+      Drude    J+ = C1 J + C0 (E+ + E) / 2,   P+ - P = dt (J+ + J) / 2;
   * sources add after the E update, monitors sample after sources
     (src/prismo/core/simulation.py:147-164).
 """
@@ -133,7 +136,8 @@ class YeeOracle:
         (i0, i1), (j0, j1), (k0, k1) = box
         shp = (i1 - i0, j1 - j0) if self.is_2d else (i1 - i0, j1 - j0, k1 - k0)
         self.ade.append(dict(comp=comp, box=box, kind=kind, eps_inf=eps_inf, c=c,
-                             P=np.zeros(shp, self.dtype), Pm=np.zeros(shp, self.dtype)))
+                             P=np.zeros(shp, self.dtype), Pm=np.zeros(shp, self.dtype),
+                             Em=np.zeros(shp, self.dtype)))
 
     def _sl(self, box):
         (i0, i1), (j0, j1), (k0, k1) = box
@@ -197,15 +201,19 @@ class YeeOracle:
             einf = r["eps_inf"]
             c0, c1, c2, c3 = r["c"]
             if r["kind"] == 1:
-                P, Pm = r["P"], r["Pm"]
-                en = ((einf - c1) * e + (1 - c2) * P - c3 * Pm + s) / (einf + c0)
-                Pn = c0 * en + c1 * e + c2 * P + c3 * Pm
-                r["Pm"], r["P"] = P, Pn
+                # keep P-, E- and solve D+ = D + s with
+                # P+ = C2 P + C3 P- + (C0 E+ + (C0 + C1) E + C1 E-) / 2
+                P, Pm, Em = r["P"], r["Pm"], r["Em"]
+                known = c2 * P + c3 * Pm + 0.5 * ((c0 + c1) * e + c1 * Em)
+                en = (einf * e + P + s - known) / (einf + 0.5 * c0)
+                Pn = known + 0.5 * c0 * en
+                r["Pm"], r["P"], r["Em"] = P, Pn, e.copy()
             else:
                 J = r["P"]
                 h = 0.5 * self.dt
-                en = (einf * e + s - h * (1 + c1) * J) / (einf + h * c0)
-                r["P"] = c0 * en + c1 * J
+                # eps_inf (E+ - E) + h (J+ + J) = s,  J+ = C1 J + C0 (E+ + E) / 2
+                en = (einf * e + s - h * J - h * (c1 * J + 0.5 * c0 * e)) / (einf + 0.5 * h * c0)
+                r["P"] = c1 * J + 0.5 * c0 * (en + e)
             new[sl] = en
         E[...] = new
 

@@ -261,8 +261,8 @@ class StepEngine:
             (i0, i1), (j0, j1), (k0, k1) = r.box
             cells = max(0, i1 - i0) * max(0, j1 - j0) * max(0, k1 - k0)
             P = self._alloc((max(cells, 1),), self.tdtype)
-            Pm = self._alloc((max(cells, 1),), self.tdtype) if r.kind == 1 else None
-            self.ade_state.append((P, Pm))
+            Q = self._alloc((max(cells, 1),), self.tdtype) if r.kind == 1 else None
+            self.ade_state.append((P, Q))
             a = arr[i]
             a.comp = ("Ex", "Ey", "Ez").index(r.comp)
             a.kind = r.kind
@@ -271,7 +271,7 @@ class StepEngine:
             a.c0, a.c1, a.c2, a.c3 = r.c
             a.cb_scale = self.dt / EPS0
             a.p = P.data_ptr()
-            a.p_prev = Pm.data_ptr() if Pm is not None else None
+            a.p_prev = Q.data_ptr() if Q is not None else None
         self._ade_arr = arr
         p.n_ade = len(regions)
         p.ade = C.cast(arr, C.POINTER(N.pm_ade_box))
@@ -434,10 +434,10 @@ class StepEngine:
         caller's)."""
         for t in self._psi_tensors():
             t.zero_()
-        for P, Pm in getattr(self, "ade_state", []):
+        for P, Q in getattr(self, "ade_state", []):
             P.zero_()
-            if Pm is not None:
-                Pm.zero_()
+            if Q is not None:
+                Q.zero_()
         for b in self._bindings.values():
             if b.acc is not None:
                 for t in b.acc:

@@ -87,6 +87,52 @@ bool box_ok(int i0, int i1, int j0, int j1, int k0, int k1, const pm_plan& p) {
          i1 <= p.nx && j1 <= p.ny && k1 <= p.nz;
 }
 
+// Kernel form of one dispersive box.  The box carries the reference's
+// coefficients (src/prismo/materials/dispersion.py:189-231, 267-286); used
+// as they stand (P+ = c0 E+ + c1 E + ..., J+ = c0 E+ + c1 J) the drive term
+// sits half a step after the P/J recursion it feeds, which shifts the phase
+// of chi(w) by w dt / 2 and shows up as a spurious gain Im(chi) ~ -Re(chi)
+// w dt / 2.  The kernels average the drive over the two half steps instead:
+//   Lorentz: P+ = c2 P + c3 P- + [(c0 E+ + c1 E) + (c0 E + c1 E-)] / 2
+//            carried as P+ = a0 E+ + a1 E + a2 P + Q,  Q+ = a3 P + a4 E
+//   Drude:   J+ = c1 J + c0 (E+ + E) / 2,  P+ = P + dt (J+ + J) / 2
+// which keeps the static limit and leaves an O((w dt)^2) error against the
+// closed-form eps(w).  E+ follows from eps_inf (E+ - E) + (P+ - P) = s with
+// s = cb_scale curlH, P and J in units of eps0.
+KAde centred_ade(const pm_ade_box& a, double dt) {
+  KAde k{};
+  k.comp = a.comp;
+  k.kind = a.kind;
+  k.i0 = a.i0; k.i1 = a.i1; k.j0 = a.j0; k.j1 = a.j1; k.k0 = a.k0; k.k1 = a.k1;
+  k.p = a.p;
+  const double einf = a.eps_inf;
+  if (a.kind == 1) {
+    k.q = a.p_prev;
+    k.a0 = 0.5 * a.c0;
+    k.a1 = 0.5 * (a.c0 + a.c1);
+    k.a2 = a.c2;
+    k.a3 = a.c3;
+    k.a4 = 0.5 * a.c1;
+    const double den = einf + k.a0;
+    k.ke = (einf - k.a1) / den;
+    k.kp = (1.0 - k.a2) / den;
+    k.kq = 1.0 / den;
+    k.ks = a.cb_scale / den;
+  } else {
+    const double h = 0.5 * dt;
+    k.q = nullptr;
+    k.a0 = 0.5 * a.c0;
+    k.a1 = 0.5 * a.c0;
+    k.a2 = a.c1;
+    const double den = einf + h * k.a0;
+    k.ke = (einf - h * k.a0) / den;
+    k.kp = -h * (1.0 + a.c1) / den;
+    k.kq = 0.0;
+    k.ks = a.cb_scale / den;
+  }
+  return k;
+}
+
 int validate(pm_ctx* c, const pm_plan& p) {
   if (p.abi_version != PM_ABI_VERSION)
     return fail(c, PM_EINVAL, "plan abi_version %d != library %d", p.abi_version,
@@ -165,6 +211,8 @@ int validate(pm_ctx* c, const pm_plan& p) {
       return fail(c, PM_EINVAL, "ADE box %d: bad component/kind", b);
     if (!box_ok(a.i0, a.i1, a.j0, a.j1, a.k0, a.k1, p) || !a.p || (a.kind == 1 && !a.p_prev))
       return fail(c, PM_EINVAL, "ADE box %d: bad extent or state", b);
+    if (!(a.eps_inf > 0.0) || !(a.cb_scale > 0.0))
+      return fail(c, PM_EINVAL, "ADE box %d: eps_inf and cb_scale must be positive", b);
   }
   if (p.n_src < 0 || p.n_src > kMaxSrc) return fail(c, PM_EINVAL, "too many sources");
   for (int s = 0; s < p.n_src; ++s) {
@@ -397,8 +445,7 @@ int pm_configure(pm_ctx* c, const pm_plan* plan) {
     KAde tab[kMaxAde];
     for (int b = 0; b < p.n_ade; ++b) {
       const pm_ade_box& a = p.ade[b];
-      tab[b] = KAde{a.comp, a.kind, a.i0, a.i1, a.j0, a.j1, a.k0, a.k1,
-                    a.eps_inf, a.c0, a.c1, a.c2, a.c3, a.cb_scale, a.p, a.p_prev};
+      tab[b] = centred_ade(a, p.dt);
     }
     // stream-ordered upload into caller memory; the host copy lives in ctx
     c->ade_host.assign(tab, tab + p.n_ade);

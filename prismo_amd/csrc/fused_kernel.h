@@ -104,8 +104,8 @@ struct AdeOut {
 };
 
 template <typename T>
-__device__ __noinline__ AdeOut<T> fade(const KAde* tab, int n_ade, double dt, int comp, int i,
-                                       int j, int k, T curl, T e) {
+__device__ __noinline__ AdeOut<T> fade(const KAde* tab, int n_ade, int comp, int i, int j,
+                                       int k, T curl, T e) {
 #pragma unroll 1
   for (int b = 0; b < n_ade; ++b) {
     const KAde& a = tab[b];
@@ -115,25 +115,7 @@ __device__ __noinline__ AdeOut<T> fade(const KAde* tab, int n_ade, double dt, in
     const int64_t l = (int64_t)(i - a.i0) +
                       (int64_t)(a.i1 - a.i0) *
                           ((int64_t)(j - a.j0) + (int64_t)(a.j1 - a.j0) * (k - a.k0));
-    const T s = (T)a.cb_scale * curl;
-    const T einf = (T)a.eps_inf;
-    T* P = (T*)a.p;
-    if (a.kind == 1) {
-      T* Pm = (T*)a.p_prev;
-      const T p0 = P[l], pm = Pm[l];
-      const T en = ((einf - (T)a.c1) * e + ((T)1 - (T)a.c2) * p0 - (T)a.c3 * pm + s) /
-                   (einf + (T)a.c0);
-      P[l] = (T)a.c0 * en + (T)a.c1 * e + (T)a.c2 * p0 + (T)a.c3 * pm;
-      Pm[l] = p0;
-      e = en;
-    } else {
-      const T J = P[l];
-      const T hdt = (T)(0.5 * dt);
-      const T en = (einf * e + s - hdt * ((T)1 + (T)a.c1) * J) / (einf + hdt * (T)a.c0);
-      P[l] = (T)a.c0 * en + (T)a.c1 * J;
-      e = en;
-    }
-    return AdeOut<T>{e, true};
+    return AdeOut<T>{ade_cell<T>(a, l, curl, e), true};
   }
   return AdeOut<T>{e, false};
 }
@@ -577,15 +559,15 @@ __global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, 
         }
         if (ade) {
           if (exok) {
-            const AdeOut<T> r = fade<T>(g.ade, g.n_ade, g.dt, 0, i, j, kk, cx, exm[e]);
+            const AdeOut<T> r = fade<T>(g.ade, g.n_ade, 0, i, j, kk, cx, exm[e]);
             if (r.hit) vx = r.e;
           }
           if (eyok) {
-            const AdeOut<T> r = fade<T>(g.ade, g.n_ade, g.dt, 1, i, j, kk, cy, eym[e]);
+            const AdeOut<T> r = fade<T>(g.ade, g.n_ade, 1, i, j, kk, cy, eym[e]);
             if (r.hit) vy = r.e;
           }
           if (ezok) {
-            const AdeOut<T> r = fade<T>(g.ade, g.n_ade, g.dt, 2, i, j, kk, cz, ezm[e]);
+            const AdeOut<T> r = fade<T>(g.ade, g.n_ade, 2, i, j, kk, cz, ezm[e]);
             if (r.hit) vz = r.e;
           }
         }
@@ -645,7 +627,7 @@ __global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, 
       if constexpr (CK == PM_COEFF_FULL) vz = ((const T*)g.ca[2])[o + e] * ezm[e] + cbz[e] * cz;
       else vz = ezm[e] + cbz[e] * cz;
       if (ade && ezok) {
-        const AdeOut<T> r = fade<T>(g.ade, g.n_ade, g.dt, 2, i, j, kk, cz, ezm[e]);
+        const AdeOut<T> r = fade<T>(g.ade, g.n_ade, 2, i, j, kk, cz, ezm[e]);
         if (r.hit) vz = r.e;
       }
       oz[e] = ezok ? vz : (T)0;

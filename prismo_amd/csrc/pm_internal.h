@@ -18,11 +18,14 @@ constexpr int kMaxDft = 8;
 constexpr int kBX = 64;
 constexpr int kBY = 4;
 
+// Dispersive box as the kernels use it: the time-centred coupled update
+// derived at configure from the reference's coefficients (see ade_cell).
 struct KAde {
   int comp, kind, i0, i1, j0, j1, k0, k1;
-  double eps_inf, c0, c1, c2, c3, cb_scale;
-  void* p;
-  void* p_prev;
+  double ke, kp, kq, ks;      // E+ = ke E + kp P - kq Q + ks curlH
+  double a0, a1, a2, a3, a4;  // P+ = a0 E+ + a1 E + a2 P + Q;  Q+ = a3 P + a4 E
+  void* p;                    // P (Lorentz) or J (Drude), box order
+  void* q;                    // Q (Lorentz only)
 };
 
 struct KSrc {
@@ -76,6 +79,26 @@ struct KGrid {
 // scratch layout (caller-owned, PM_SCRATCH_BYTES)
 constexpr int kScratchFlags = 0;
 constexpr int kScratchAde = 1024;
+
+// One dispersive cell: new E from the old E and the curl of H, advancing the
+// polarisation state at linear index l of the box.  Drude boxes have kq = 0,
+// a3 = a4 = 0 and no Q array.
+template <typename T>
+__device__ __forceinline__ T ade_cell(const KAde& a, int64_t l, T curl, T e) {
+  T* P = (T*)a.p;
+  const T p0 = P[l];
+  if (a.kind == 1) {
+    T* Q = (T*)a.q;
+    const T q0 = Q[l];
+    const T en = (T)a.ke * e + (T)a.kp * p0 - (T)a.kq * q0 + (T)a.ks * curl;
+    P[l] = (T)a.a0 * en + (T)a.a1 * e + (T)a.a2 * p0 + q0;
+    Q[l] = (T)a.a3 * p0 + (T)a.a4 * e;
+    return en;
+  }
+  const T en = (T)a.ke * e + (T)a.kp * p0 + (T)a.ks * curl;
+  P[l] = (T)a.a0 * en + (T)a.a1 * e + (T)a.a2 * p0;
+  return en;
+}
 
 struct KAux {
   int nx, ny, nz;

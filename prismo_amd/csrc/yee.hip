@@ -64,10 +64,9 @@ __device__ __forceinline__ T load_cb(const KGrid& g, int c, int64_t off) {
   }
 }
 
-// Dispersive E update.  With P, J scaled by 1/eps0 and s = (dt/eps0) curl H:
-//   Lorentz: eps_inf (E+ - E) + (P+ - P) = s,  P+ = c0 E+ + c1 E + c2 P + c3 P-
-//   Drude:   eps_inf (E+ - E) = s - dt (J+ + J) / 2,  J+ = c0 E+ + c1 J
-// each solved for E+ in closed form.  Returns true if (i,j,k) is in a box.
+// Dispersive E update: the first box of the table holding (comp, i, j, k)
+// replaces the plain update (ade_cell in pm_internal.h).  Returns true if the
+// cell is dispersive.
 template <typename T>
 __device__ __forceinline__ bool ade_update(const KGrid& g, int comp, int i, int j,
                                            int k, T curl, T& e) {
@@ -82,24 +81,7 @@ __device__ __forceinline__ bool ade_update(const KGrid& g, int comp, int i, int 
     const int64_t l = (int64_t)(i - a.i0) +
                       (int64_t)(a.i1 - a.i0) *
                           ((int64_t)(j - a.j0) + (int64_t)(a.j1 - a.j0) * (k - a.k0));
-    const T s = (T)a.cb_scale * curl;
-    const T einf = (T)a.eps_inf;
-    T* P = (T*)a.p;
-    if (a.kind == 1) {
-      T* Pm = (T*)a.p_prev;
-      const T p0 = P[l], pm = Pm[l];
-      const T en = ((einf - (T)a.c1) * e + ((T)1 - (T)a.c2) * p0 - (T)a.c3 * pm + s) /
-                   (einf + (T)a.c0);
-      P[l] = (T)a.c0 * en + (T)a.c1 * e + (T)a.c2 * p0 + (T)a.c3 * pm;
-      Pm[l] = p0;
-      e = en;
-    } else {
-      const T J = P[l];
-      const T hdt = (T)(0.5 * g.dt);
-      const T en = (einf * e + s - hdt * ((T)1 + (T)a.c1) * J) / (einf + hdt * (T)a.c0);
-      P[l] = (T)a.c0 * en + (T)a.c1 * J;
-      e = en;
-    }
+    e = ade_cell<T>(a, l, curl, e);
     return true;
   }
   return false;

@@ -135,24 +135,29 @@ class SellmeierMaterial(DispersiveMaterial):
 
 
 def discrete_susceptibility(material, dt: float, omega):
-    """Exact response eps_num(w) = eps_inf + chi(z), z = exp(i w dt), of the
-    coupled recursions the E kernel runs (time convention exp(-i w t) as the
-    closed forms above).
+    """Exact response eps_num(w) = eps_inf + chi(z) of the coupled
+    recursions the E kernels run, z^-1 = exp(i w dt) (time convention
+    exp(-i w t), as the closed forms above).
 
-    Lorentz: P+ = c0 E+ + c1 E + c2 P + c3 P-  ->  chi = (c0 + c1 z^-1) /
-             (1 - c2 z^-1 - c3 z^-2) with z = exp(-i w dt) for exp(-i w t).
-    Drude:   J+ = c0 E+ + c1 J and eps_inf (E+ - E) + dt (J+ + J)/2 = s,
-             D = eps_inf E + P with (P+ - P) = dt (J+ + J)/2, so
-             chi = dt/2 (1 + z^-1) c0 / ((1 - c1 z^-1)(1 - z^-1)).
+    The kernels feed the coefficients above with the drive averaged over the
+    half steps on either side of each update:
+    Lorentz: P+ = C2 P + C3 P- + [(C0 E+ + C1 E) + (C0 E + C1 E-)] / 2
+             -> chi = (C0 + C1 z^-1)(1 + z^-1) / 2 / (1 - C2 z^-1 - C3 z^-2)
+    Drude:   J+ = C1 J + C0 (E+ + E) / 2,  P+ - P = dt (J+ + J) / 2
+             -> chi = dt/2 (1 + z^-1)/(1 - z^-1) * C0 (1 + z^-1) / 2 / (1 - C1 z^-1)
+    Both keep chi(w -> 0) of the closed form and differ from it by
+    O((w dt)^2); the uncentred use (drive C0 E+ + C1 E alone) would carry an
+    extra phase exp(-i w dt / 2), i.e. a spurious gain.
     """
     zi = np.exp(1j * np.asarray(omega) * dt)  # z^-1 for exp(-i w t)
     if isinstance(material, LorentzMaterial):
         chi = 0
         for p in material.get_ade_coefficients(dt)["poles"]:
-            chi = chi + (p["C0"] + p["C1"] * zi) / (1 - p["C2"] * zi - p["C3"] * zi**2)
+            chi = chi + 0.5 * (p["C0"] + p["C1"] * zi) * (1 + zi) / (
+                1 - p["C2"] * zi - p["C3"] * zi**2)
         return material.epsilon_inf + chi
     if isinstance(material, DrudeMaterial):
         co = material.get_ade_coefficients(dt)
-        chi = 0.5 * dt * (1 + zi) * co["C0"] / ((1 - co["C1"] * zi) * (1 - zi))
+        chi = 0.25 * dt * (1 + zi) ** 2 * co["C0"] / ((1 - co["C1"] * zi) * (1 - zi))
         return material.epsilon_inf + chi
     raise TypeError(f"no discrete model for {type(material).__name__}")
