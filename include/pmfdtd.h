@@ -244,10 +244,11 @@ int pm_fused_active(pm_ctx* ctx);
 /* Block until all work on the context's stream has finished. */
 int pm_sync(pm_ctx* ctx);
 
-/* Average kernel time (ms) of the last pm_run, measured with HIP events on
- * the context stream around every launch of kernel class `which`
- * (0 = H update, 1 = E update, 2 = whole step, 3 = fused H+E step);
- * enabled by pm_set_timing. */
+/* Average kernel time (ms) since timing was enabled, measured with HIP
+ * events on the context stream around every launch of kernel class `which`
+ * (0 = H update, 1 = E update, 2 = whole step, 3 = fused H+E step,
+ * 4 = spectral flush of one sample block); enabled (and reset) by
+ * pm_set_timing. */
 int pm_set_timing(pm_ctx* ctx, int enable);
 int pm_get_timing(pm_ctx* ctx, int which, double* avg_ms, int64_t* count);
 

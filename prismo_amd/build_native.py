@@ -35,6 +35,7 @@ UNITS = {
     "yee.hip": [],
     "compat.hip": ["-ffp-contract=off"],
     "aux.hip": [],
+    "dft_mfma.hip": [],
     "fused_f32.hip": [],
     "fused_f64.hip": [],
 }

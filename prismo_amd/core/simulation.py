@@ -148,10 +148,11 @@ class Simulation:
         esz = 8 if eng.precision == N.FP64 else 4
         ncols = sum(int(np.prod(sh)) for m in dev_mons for _, _, sh in m.device_spec()["boxes"])
         any_td = any(m.device_spec()["time_domain"] for m in dev_mons)
+        # rows per spectral block: FP32 sums each block on the matrix cores
+        # before the compensated add, so its blocks stay short
+        T = 128 if esz == 4 else 512
         if any_td and ncols:
-            T = int(max(1, min(steps, (256 << 20) // max(1, ncols * esz))))
-        else:
-            T = 128
+            T = int(max(1, min(T, steps, (256 << 20) // max(1, ncols * esz))))
         step0 = self.step_count
         eng.configure_io(terms, dev_mons, times_arr, step0=step0, sample_T=T)
         for m in dev_mons:

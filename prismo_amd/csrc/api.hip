@@ -32,7 +32,7 @@ struct pm_ctx {
   bool timing = false;
   std::vector<hipEvent_t> pool;
   size_t pool_used = 0;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> spans[4];
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> spans[5];
 };
 
 namespace {
@@ -339,6 +339,7 @@ int flush_pending(pm_ctx* c) {
                 (long long)c->pend0, (long long)(c->pend0 + c->pend_n - 1));
   const int esz = p.precision == PM_FP64 ? 8 : 4;
   const char* X0 = (const char*)p.X + (c->pend0 % p.sample_T) * (int64_t)p.ncols * esz;
+  Span sp(c, 4);
   for (const KDft& d : c->dft)
     launch_dft(d, X0, p.ncols, p.W_re, p.W_im, p.table_len, wcol0, c->pend_n, p.precision,
                c->stream);
@@ -621,7 +622,7 @@ int pm_set_timing(pm_ctx* c, int enable) {
 }
 
 int pm_get_timing(pm_ctx* c, int which, double* avg_ms, int64_t* count) {
-  if (!c || which < 0 || which > 3 || !avg_ms || !count) return PM_EINVAL;
+  if (!c || which < 0 || which > 4 || !avg_ms || !count) return PM_EINVAL;
   int r = hip_check(c, hipStreamSynchronize(c->stream), "stream sync");
   if (r) return r;
   double tot = 0;

@@ -1,5 +1,5 @@
-// Source injection, monitor sampling and the scalar spectral-accumulation
-// fallback kernel (the MFMA accumulation lives in dft_mfma.hip).
+// Source injection and monitor sampling (the spectral accumulation of the
+// sampled blocks is in dft_mfma.hip).
 //
 // Sources: field[c] += value on a box (src/prismo/sources/point.py:66-73,
 // plane_wave.py:206-221, gaussian.py:270-298).  For FP32 fields the value is
@@ -87,69 +87,6 @@ void launch_sample(const KAux& a, int64_t step, hipStream_t s) {
                          a.sz, (const float*)a.F[b.comp],
                          (float*)a.X + row * (int64_t)a.ncols);
   }
-}
-
-// Scalar accumulation: one thread per (f, p), sum over the block's rows in
-// the field precision, then add into the accumulator (FP32: compensated).
-template <typename T>
-__global__ __launch_bounds__(256) void k_dft_scalar(const KDft d, const T* X, int ncols,
-                                                    const T* Wre, const T* Wim,
-                                                    int64_t table_len, int64_t wcol0,
-                                                    int rows) {
-  const int np = d.col1 - d.col0;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  const int f = blockIdx.y;
-  if (p >= np) return;
-  const T* wr = Wre + (int64_t)(d.f0 + f) * table_len + wcol0;
-  const T* wi = Wim + (int64_t)(d.f0 + f) * table_len + wcol0;
-  T sr = 0, si = 0;
-  for (int n = 0; n < rows; ++n) {
-    const T x = X[(int64_t)n * ncols + d.col0 + p];
-    sr += wr[n] * x;
-    si += wi[n] * x;
-  }
-  const int64_t o = (int64_t)f * np + p;
-  T* ar = (T*)d.acc_re;
-  T* ai = (T*)d.acc_im;
-  if (d.cmp_re == nullptr) {
-    ar[o] += sr;
-    ai[o] += si;
-  } else {  // Kahan-Babuska: acc + cmp carries the running sum
-    T* cr = (T*)d.cmp_re;
-    T* ci = (T*)d.cmp_im;
-    {
-      const T a = ar[o], t = a + sr;
-      const T e = (fabs(a) >= fabs(sr)) ? (a - t) + sr : (sr - t) + a;
-      ar[o] = t;
-      cr[o] += e;
-    }
-    {
-      const T a = ai[o], t = a + si;
-      const T e = (fabs(a) >= fabs(si)) ? (a - t) + si : (si - t) + a;
-      ai[o] = t;
-      ci[o] += e;
-    }
-  }
-}
-
-void launch_dft_scalar(const KDft& d, const void* X, int ncols, const void* W_re,
-                       const void* W_im, int64_t table_len, int64_t wcol0, int rows,
-                       int precision, hipStream_t s) {
-  const int np = d.col1 - d.col0, nf = d.f1 - d.f0;
-  if (np <= 0 || nf <= 0 || rows <= 0) return;
-  dim3 grid((np + 255) / 256, nf);
-  if (precision == PM_FP64)
-    hipLaunchKernelGGL(k_dft_scalar<double>, grid, dim3(256), 0, s, d, (const double*)X,
-                       ncols, (const double*)W_re, (const double*)W_im, table_len, wcol0,
-                       rows);
-  else
-    hipLaunchKernelGGL(k_dft_scalar<float>, grid, dim3(256), 0, s, d, (const float*)X, ncols,
-                       (const float*)W_re, (const float*)W_im, table_len, wcol0, rows);
-}
-
-void launch_dft(const KDft& d, const void* X, int ncols, const void* W_re, const void* W_im,
-                int64_t table_len, int64_t wcol0, int rows, int precision, hipStream_t s) {
-  launch_dft_scalar(d, X, ncols, W_re, W_im, table_len, wcol0, rows, precision, s);
 }
 
 }  // namespace pm
