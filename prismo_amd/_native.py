@@ -94,7 +94,57 @@ class pm_plan(C.Structure):
     ]
 
 
-# Exported symbols and their signatures (the full set declared in the header).
+# include/pmbatch.h
+BATCH_MAX_LINES = 8
+
+
+class pm_batch_line(C.Structure):
+    _fields_ = [("comp", i32), ("i", i32), ("j0", i32), ("j1", i32), ("col", i32), ("pad", i32)]
+
+
+class pm_batch_design(C.Structure):
+    _fields_ = [
+        ("nx", i32), ("ny", i32), ("sy", i32), ("n_lines", i32),
+        ("pml", i32 * 4),
+        ("ez", vp * 2), ("hx", vp * 2), ("hy", vp * 2),
+        ("psi_hx", vp * 2), ("psi_hy", vp * 2), ("psi_ex", vp), ("psi_ey", vp),
+        ("cb", vp), ("cb_scale", f64), ("db", f64), ("prof", vp),
+        ("src_e", i32 * 3), ("src_h", i32 * 3),
+        ("line", pm_batch_line * BATCH_MAX_LINES),
+    ]
+
+
+class pm_batch_plan(C.Structure):
+    _fields_ = [
+        ("abi_version", i32), ("precision", i32), ("coeff_kind", i32), ("n_design", i32),
+        ("dx", f64), ("dy", f64), ("dt", f64),
+        ("design", C.POINTER(pm_batch_design)),
+        ("src_e_table", vp), ("src_h_table", vp),
+        ("table_step0", i64), ("table_len", i64),
+        ("X", vp), ("ncols", i32), ("sample_T", i32),
+        ("dft", pm_dft_group),
+        ("W_re", vp), ("W_im", vp),
+        ("nfreq", i32), ("chunk", i32),
+        ("work", vp), ("work_bytes", i64),
+    ]
+
+
+# Exported symbols and their signatures (the full set declared in the headers).
+BATCH_SIGNATURES = {
+    "pm_batch_create": ([C.c_int, vp, C.POINTER(vp)], C.c_int),
+    "pm_batch_destroy": ([vp], C.c_int),
+    "pm_batch_last_error": ([vp], C.c_char_p),
+    "pm_batch_workspace_bytes": ([C.POINTER(pm_batch_plan), C.POINTER(i64)], C.c_int),
+    "pm_batch_configure": ([vp, C.POINTER(pm_batch_plan)], C.c_int),
+    "pm_batch_run": ([vp, i32, C.c_uint32], C.c_int),
+    "pm_batch_flush": ([vp], C.c_int),
+    "pm_batch_parity": ([vp], C.c_int),
+    "pm_batch_steps": ([vp], i64),
+    "pm_batch_sync": ([vp], C.c_int),
+    "pm_batch_set_timing": ([vp, C.c_int], C.c_int),
+    "pm_batch_get_timing": ([vp, C.c_int, C.POINTER(f64), C.POINTER(i64)], C.c_int),
+}
+
 SIGNATURES = {
     "pm_abi_version": ([], C.c_int),
     "pm_create": ([C.c_int, vp, C.POINTER(vp)], C.c_int),
@@ -131,7 +181,7 @@ def load(path: os.PathLike | None = None):
             "(the mi355x time step has no CPU fallback)"
         )
     lib = C.CDLL(str(p))
-    for name, (args, res) in SIGNATURES.items():
+    for name, (args, res) in {**SIGNATURES, **BATCH_SIGNATURES}.items():
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = res
@@ -206,6 +256,78 @@ class Context:
     def close(self):
         if getattr(self, "h", None):
             self.lib.pm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BatchContext:
+    """One pm_batch on a device/stream (include/pmbatch.h)."""
+
+    def __init__(self, device: int = 0, stream: int | None = None):
+        self.lib = load()
+        h = vp()
+        rc = self.lib.pm_batch_create(device, vp(stream or 0), C.byref(h))
+        if rc != 0:
+            raise NativeError(f"pm_batch_create failed ({rc})")
+        self.h = h
+        self._keep = []
+
+    def _check(self, rc: int, what: str):
+        if rc == 0:
+            return
+        msg = self.lib.pm_batch_last_error(self.h).decode(errors="replace")
+        kind = ERRORS.get(rc, f"HIP error {rc}")
+        if rc in (-1, -4):
+            raise ValueError(f"{what}: {kind}: {msg}")
+        if rc == -2:
+            raise NotImplementedError(f"{what}: {kind}: {msg}")
+        raise NativeError(f"{what}: {kind}: {msg}")
+
+    def workspace_bytes(self, plan: pm_batch_plan) -> int:
+        n = i64()
+        rc = self.lib.pm_batch_workspace_bytes(C.byref(plan), C.byref(n))
+        if rc != 0:
+            raise ValueError("pm_batch_workspace_bytes: invalid plan")
+        return n.value
+
+    def configure(self, plan: pm_batch_plan, keep=()):
+        self._keep = list(keep)
+        self._check(self.lib.pm_batch_configure(self.h, C.byref(plan)), "pm_batch_configure")
+
+    def run(self, nsteps: int, graph: bool = False):
+        self._check(self.lib.pm_batch_run(self.h, nsteps, RUN_GRAPH if graph else 0),
+                    "pm_batch_run")
+
+    def flush(self):
+        self._check(self.lib.pm_batch_flush(self.h), "pm_batch_flush")
+
+    def parity(self) -> int:
+        return int(self.lib.pm_batch_parity(self.h))
+
+    def steps(self) -> int:
+        return int(self.lib.pm_batch_steps(self.h))
+
+    def sync(self):
+        self._check(self.lib.pm_batch_sync(self.h), "pm_batch_sync")
+
+    def set_timing(self, on: bool):
+        self._check(self.lib.pm_batch_set_timing(self.h, 1 if on else 0), "pm_batch_set_timing")
+
+    def timing(self, which: int):
+        ms = f64()
+        n = i64()
+        self._check(self.lib.pm_batch_get_timing(self.h, which, C.byref(ms), C.byref(n)),
+                    "pm_batch_get_timing")
+        return ms.value, n.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.pm_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -36,6 +36,7 @@ UNITS = {
     "compat.hip": ["-ffp-contract=off"],
     "aux.hip": [],
     "dft_mfma.hip": [],
+    "batch2d.hip": [],
     "fused_f32.hip": [],
     "fused_f64.hip": [],
 }

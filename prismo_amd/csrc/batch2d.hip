@@ -1,0 +1,605 @@
+// Batched 2-D TM stepping of many uneven grids (include/pmbatch.h).
+//
+// One launch advances every design by one step.  The work list holds one
+// tile per wave: (design, x0, r0, r1) = a strip of 62 owned x cells
+// [x0, x0+62) and the rows [r0, r1).  The wave's 64 lanes cover x0-1 ..
+// x0+62: lane 0 and lane 63 are halo lanes that only provide Ez(i-1) and
+// Hy(i+1) to their neighbours through lane shuffles.  The wave marches up
+// its rows once, carrying row j-1 in registers:
+//   H(n+1/2) at row j   from Ez(j), Ez(j-1) (registers) and Ez(i-1) (shuffle);
+//   E(n+1)   at row j-1 from Hx(j), Hx(j-1), Hy(i), Hy(i+1) (registers/shuffle),
+// so every array is read once and written once per step (plus one halo row
+// per tile and two halo columns per strip).  Reads come from field set
+// `par`, writes go to set par^1, so neighbouring tiles never see a
+// half-updated value; the H-side CPML memory is ping-ponged for the same
+// reason, the E-side memory is touched only by the owner and stays in place.
+// Sources are added after the E update, samples are taken after the sources,
+// as src/prismo/core/simulation.py:147-164 orders them.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/pmbatch.h"
+#include "pm_internal.h"
+
+namespace pm {
+namespace {
+
+constexpr int kStrip = 62;   // owned cells per wave
+constexpr int kWaves = 4;    // waves per workgroup
+
+struct KWork {
+  int d, x0, r0, r1;
+};
+
+struct KBatch {
+  const pm_batch_design* des;
+  const KWork* work;
+  int n_work;
+  int ncols, sample_T;
+  int64_t* clock;
+  double idx, idy;
+  const double* tab_e;
+  const double* tab_h;
+  int64_t table_step0;
+  void* X;
+};
+
+template <typename T, int CK>
+__device__ __forceinline__ T cb_at(const pm_batch_design& D, int64_t o) {
+  if constexpr (CK == PM_COEFF_CB_F16) {
+    return (T)D.cb_scale * (T)__half2float(((const __half*)D.cb)[o]);
+  } else {
+    return ((const T*)D.cb)[o];
+  }
+}
+
+template <typename T, int CK>
+__global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_t step_arg,
+                                                          int use_clock, int s_rel, int par) {
+  const int lane = threadIdx.x & 63;
+  const int item = blockIdx.x * kWaves + (int)(threadIdx.x >> 6);
+  if (item >= k.n_work) return;  // whole wave
+  const KWork w = k.work[item];
+  const pm_batch_design& D = k.des[w.d];
+  const int64_t step = use_clock ? (*k.clock + s_rel) : step_arg;
+  const int nx = D.nx, ny = D.ny;
+  const int64_t sy = D.sy;
+  const int i = w.x0 - 1 + lane;
+  const bool inx = i >= 0 && i < nx;
+  const bool own = lane >= 1 && lane <= kStrip && inx;
+  const int q = par, qn = par ^ 1;
+  const T* EzA = (const T*)D.ez[q];
+  const T* HxA = (const T*)D.hx[q];
+  const T* HyA = (const T*)D.hy[q];
+  T* EzB = (T*)D.ez[qn];
+  T* HxB = (T*)D.hx[qn];
+  T* HyB = (T*)D.hy[qn];
+  const T* psHxA = (const T*)D.psi_hx[q];
+  const T* psHyA = (const T*)D.psi_hy[q];
+  T* psHxB = (T*)D.psi_hx[qn];
+  T* psHyB = (T*)D.psi_hy[qn];
+  T* psEx = (T*)D.psi_ex;
+  T* psEy = (T*)D.psi_ey;
+  const T* prof = (const T*)D.prof;
+  const T* py = prof + 6 * (int64_t)nx;
+
+  // x profiles of this lane (identity outside the slabs)
+  const bool xpml = inx && (i < D.pml[0] || i >= nx - D.pml[1]);
+  T bex = 1, cex = 0, ikex = 1, bhx = 1, chx = 0, ikhx = 1;
+  if (xpml) {
+    bex = prof[i];
+    cex = prof[nx + i];
+    ikex = prof[2 * nx + i];
+    bhx = prof[3 * nx + i];
+    chx = prof[4 * nx + i];
+    ikhx = prof[5 * nx + i];
+  }
+  const T db = (T)D.db;
+  const T idx = (T)k.idx, idy = (T)k.idy;
+  const int64_t trow = step - k.table_step0;
+  const T se = (T)k.tab_e[trow];
+  const T sh = (T)k.tab_h[trow];
+  T* Xrow = (T*)k.X + (step % k.sample_T) * (int64_t)k.ncols;
+  const bool srcE = i == D.src_e[0];
+  const bool srcH = i == D.src_h[0];
+
+  const int r0 = w.r0, r1 = w.r1;
+  const int jend = min(r1, ny - 1);
+  T ezp = (r0 >= 1 && inx) ? EzA[(int64_t)(r0 - 1) * sy + i] : (T)0;
+  T hxp = 0, hyp = 0;
+  for (int j = r0; j <= jend; ++j) {
+    const int64_t o = (int64_t)j * sy + i;
+    const T ez = inx ? EzA[o] : (T)0;
+    const T hx = inx ? HxA[o] : (T)0;
+    const T hy = inx ? HyA[o] : (T)0;
+    const bool ypml = j < D.pml[2] || j >= ny - D.pml[3];
+    const bool store_h = own && j < r1;
+
+    // Hx(i, j) -= Db * dEz/dy (backward difference, Ez(j = -1) = 0)
+    T dy = (ez - ezp) * idy;
+    if (ypml) {
+      const T b = py[3 * ny + j], c = py[4 * ny + j], ik = py[5 * ny + j];
+      T ps = inx ? psHxA[o] : (T)0;
+      ps = b * ps + c * dy;
+      if (store_h) psHxB[o] = ps;
+      dy = dy * ik + ps;
+    }
+    const T hxn = (inx && i < nx - 1) ? hx - db * dy : (T)0;
+
+    // Hy(i, j) += Db * dEz/dx, Ez(i-1) from the lane below
+    const T ezl = __shfl_up(ez, 1);
+    T dx = (ez - ezl) * idx;
+    if (xpml) {
+      T ps = psHyA[o];
+      ps = bhx * ps + chx * dx;
+      if (store_h) psHyB[o] = ps;
+      dx = dx * ikhx + ps;
+    }
+    const T hyn = (inx && j < ny - 1) ? hy + db * dx : (T)0;
+
+    if (store_h) {
+      HxB[o] = hxn;
+      const T hys = (srcH && j >= D.src_h[1] && j < D.src_h[2]) ? hyn + sh : hyn;
+      HyB[o] = hys;
+      for (int l = 0; l < D.n_lines; ++l) {
+        const pm_batch_line& L = D.line[l];
+        if (L.comp != 0 && i == L.i && j >= L.j0 && j < L.j1)
+          Xrow[L.col + (j - L.j0)] = L.comp == 2 ? hys : hxn;
+      }
+    }
+
+    if (j > r0) {
+      // Ez(i, jj) += Cb (dHy/dx - dHx/dy) with the new H of rows jj, jj+1
+      const int jj = j - 1;
+      const int64_t oe = o - sy;
+      const T hyr = __shfl_down(hyp, 1);
+      const bool valid = inx && i < nx - 1 && jj < ny - 1;
+      T tx = (hyr - hyp) * idx;
+      if (xpml && valid) {
+        T ps = psEx[oe];
+        ps = bex * ps + cex * tx;
+        if (own) psEx[oe] = ps;
+        tx = tx * ikex + ps;
+      }
+      T ty = (hxn - hxp) * idy;
+      if ((jj < D.pml[2] || jj >= ny - D.pml[3]) && valid) {
+        const T b = py[jj], c = py[ny + jj], ik = py[2 * ny + jj];
+        T ps = psEy[oe];
+        ps = b * ps + c * ty;
+        if (own) psEy[oe] = ps;
+        ty = ty * ik + ps;
+      }
+      T ezn = (T)0;
+      if (valid) {
+        ezn = ezp + cb_at<T, CK>(D, oe) * (tx - ty);
+        if (srcE && jj >= D.src_e[1] && jj < D.src_e[2]) ezn += se;
+      }
+      if (own) {
+        EzB[oe] = ezn;
+        for (int l = 0; l < D.n_lines; ++l) {
+          const pm_batch_line& L = D.line[l];
+          if (L.comp == 0 && i == L.i && jj >= L.j0 && jj < L.j1)
+            Xrow[L.col + (jj - L.j0)] = ezn;
+        }
+      }
+    }
+    ezp = ez;
+    hxp = hxn;
+    hyp = hyn;
+  }
+}
+
+__global__ void k_clock_set(int64_t* clock, int64_t v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *clock = v;
+}
+
+__global__ void k_clock_add(int64_t* clock, int64_t v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *clock += v;
+}
+
+}  // namespace
+}  // namespace pm
+
+using namespace pm;
+
+struct pm_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool configured = false;
+  pm_batch_plan plan{};
+  std::vector<pm_batch_design> des;
+  std::vector<KWork> work;
+  KBatch kb{};
+  KDft dft{};
+  int64_t step = 0;   // absolute index of the next step
+  int parity = 0;
+  int64_t pend0 = 0;  // first step of the pending sample block
+  int pend_n = 0;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> spans[3];
+  std::string err;
+};
+
+namespace {
+
+int bfail(pm_batch* b, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (b) b->err = buf;
+  return code;
+}
+
+int bhip(pm_batch* b, hipError_t e, const char* what) {
+  if (e == hipSuccess) return 0;
+  return bfail(b, (int)e, "%s: %s", what, hipGetErrorString(e));
+}
+
+struct BSpan {
+  pm_batch* b;
+  int which;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  BSpan(pm_batch* bb, int w) : b(bb), which(w) {
+    if (b->timing && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess)
+      (void)hipEventRecord(e0, b->stream);
+  }
+  ~BSpan() {
+    if (e0 && e1) {
+      (void)hipEventRecord(e1, b->stream);
+      b->spans[which].push_back({e0, e1});
+    }
+  }
+};
+
+int64_t align256(int64_t v) { return (v + 255) & ~(int64_t)255; }
+
+void layout(const pm_batch_plan& p, const std::vector<KWork>* work, int64_t* des_off,
+            int64_t* work_off, int64_t* total, int64_t n_work) {
+  *des_off = 256;  // the clock sits at offset 0
+  *work_off = align256(*des_off + (int64_t)sizeof(pm_batch_design) * p.n_design);
+  *total = align256(*work_off + (int64_t)sizeof(KWork) * n_work);
+  (void)work;
+}
+
+void build_work(const pm_batch_plan& p, std::vector<KWork>& work) {
+  work.clear();
+  const int ch = p.chunk;
+  for (int d = 0; d < p.n_design; ++d) {
+    const pm_batch_design& D = p.design[d];
+    for (int r0 = 0; r0 < D.ny; r0 += ch)
+      for (int x0 = 0; x0 < D.nx; x0 += kStrip)
+        work.push_back(KWork{d, x0, r0, std::min(r0 + ch, D.ny)});
+  }
+}
+
+int validate(pm_batch* b, const pm_batch_plan& p) {
+  if (p.abi_version != PM_ABI_VERSION) return bfail(b, PM_EINVAL, "ABI version mismatch");
+  if (p.precision != PM_FP64 && p.precision != PM_FP32)
+    return bfail(b, PM_EINVAL, "precision must be PM_FP64 or PM_FP32");
+  if (p.coeff_kind != PM_COEFF_CB && p.coeff_kind != PM_COEFF_CB_F16)
+    return bfail(b, PM_EINVAL, "coeff_kind must be CB or CB_F16");
+  if (p.n_design < 1 || !p.design) return bfail(b, PM_EINVAL, "no designs");
+  if (!(p.dx > 0) || !(p.dy > 0) || !(p.dt > 0)) return bfail(b, PM_EINVAL, "bad dx/dy/dt");
+  if (p.chunk < 2) return bfail(b, PM_EINVAL, "chunk must be >= 2");
+  if (p.sample_T < 1 || p.ncols < 0 || !p.X) return bfail(b, PM_EINVAL, "bad sample ring");
+  if (!p.src_e_table || !p.src_h_table || p.table_len < 1)
+    return bfail(b, PM_EINVAL, "source tables required");
+  if (p.nfreq < 0) return bfail(b, PM_EINVAL, "bad nfreq");
+  if (p.nfreq > 0) {
+    if (!p.W_re || !p.W_im || !p.dft.acc_re || !p.dft.acc_im)
+      return bfail(b, PM_EINVAL, "spectra need W tables and accumulators");
+    if (p.dft.col0 < 0 || p.dft.col1 > p.ncols || p.dft.col0 >= p.dft.col1 ||
+        p.dft.f0 < 0 || p.dft.f1 > p.nfreq || p.dft.f0 >= p.dft.f1)
+      return bfail(b, PM_EINVAL, "DFT group outside the sample columns / frequencies");
+    if (p.precision == PM_FP32 && (!p.dft.cmp_re || !p.dft.cmp_im))
+      return bfail(b, PM_EINVAL, "FP32 spectra need compensation arrays");
+  }
+  for (int d = 0; d < p.n_design; ++d) {
+    const pm_batch_design& D = p.design[d];
+    if (D.nx < 2 || D.ny < 2 || D.sy < D.nx) return bfail(b, PM_EINVAL, "design %d: bad size", d);
+    for (int s = 0; s < 2; ++s)
+      if (!D.ez[s] || !D.hx[s] || !D.hy[s] || !D.psi_hx[s] || !D.psi_hy[s])
+        return bfail(b, PM_EINVAL, "design %d: missing field or CPML arrays", d);
+    if (!D.psi_ex || !D.psi_ey || !D.cb || !D.prof)
+      return bfail(b, PM_EINVAL, "design %d: missing arrays", d);
+    if (D.pml[0] < 0 || D.pml[1] < 0 || D.pml[2] < 0 || D.pml[3] < 0 ||
+        D.pml[0] + D.pml[1] > D.nx || D.pml[2] + D.pml[3] > D.ny)
+      return bfail(b, PM_EINVAL, "design %d: bad CPML layers", d);
+    if (D.n_lines < 0 || D.n_lines > PM_BATCH_MAX_LINES)
+      return bfail(b, PM_EINVAL, "design %d: too many sample lines", d);
+    for (int l = 0; l < D.n_lines; ++l) {
+      const pm_batch_line& L = D.line[l];
+      if (L.comp < 0 || L.comp > 2 || L.i < 0 || L.i >= D.nx || L.j0 < 0 || L.j1 > D.ny ||
+          L.j0 > L.j1 || L.col < 0 || L.col + (L.j1 - L.j0) > p.ncols)
+        return bfail(b, PM_EINVAL, "design %d: bad sample line %d", d, l);
+    }
+    // Ez exists on (nx-1) x (ny-1), Hy on nx x (ny-1)
+    if (D.src_e[1] < 0 || D.src_e[1] > D.src_e[2] ||
+        (D.src_e[2] > D.src_e[1] && (D.src_e[0] < 0 || D.src_e[0] >= D.nx - 1 ||
+                                     D.src_e[2] > D.ny - 1)))
+      return bfail(b, PM_EINVAL, "design %d: Ez source line outside Ez", d);
+    if (D.src_h[1] < 0 || D.src_h[1] > D.src_h[2] ||
+        (D.src_h[2] > D.src_h[1] && (D.src_h[0] < 0 || D.src_h[0] >= D.nx ||
+                                     D.src_h[2] > D.ny - 1)))
+      return bfail(b, PM_EINVAL, "design %d: Hy source line outside Hy", d);
+  }
+  return 0;
+}
+
+template <typename T>
+int launch_step(pm_batch* b, int64_t step_arg, int use_clock, int s_rel, int par) {
+  const int nb = ((int)b->work.size() + kWaves - 1) / kWaves;
+  if (b->plan.coeff_kind == PM_COEFF_CB_F16)
+    hipLaunchKernelGGL((k_batch_tm<T, PM_COEFF_CB_F16>), dim3(nb), dim3(64 * kWaves), 0,
+                       b->stream, b->kb, step_arg, use_clock, s_rel, par);
+  else
+    hipLaunchKernelGGL((k_batch_tm<T, PM_COEFF_CB>), dim3(nb), dim3(64 * kWaves), 0,
+                       b->stream, b->kb, step_arg, use_clock, s_rel, par);
+  return bhip(b, hipGetLastError(), "batch step launch");
+}
+
+int step_once(pm_batch* b, int64_t step, int use_clock, int s_rel, int par) {
+  return b->plan.precision == PM_FP64 ? launch_step<double>(b, step, use_clock, s_rel, par)
+                                      : launch_step<float>(b, step, use_clock, s_rel, par);
+}
+
+int flush_block(pm_batch* b, int64_t first, int rows, const int64_t* wclock) {
+  const pm_batch_plan& p = b->plan;
+  if (p.nfreq == 0 || rows == 0) return 0;
+  const int esz = p.precision == PM_FP64 ? 8 : 4;
+  const char* X0 = (const char*)p.X + (first % p.sample_T) * (int64_t)p.ncols * esz;
+  const int64_t wcol = wclock ? p.table_step0 : first - p.table_step0;
+  launch_dft(b->dft, X0, p.ncols, p.W_re, p.W_im, p.table_len, wcol, rows, p.precision,
+             b->stream, wclock);
+  return bhip(b, hipGetLastError(), "batch DFT launch");
+}
+
+int capture(pm_batch* b, int par) {
+  const pm_batch_plan& p = b->plan;
+  hipGraph_t g = nullptr;
+  int r = bhip(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal),
+               "begin capture");
+  if (r) return r;
+  for (int s = 0; s < p.sample_T && !r; ++s) r = step_once(b, 0, 1, s, par ^ (s & 1));
+  if (!r) r = flush_block(b, 0, p.sample_T, b->kb.clock);
+  if (!r) {
+    hipLaunchKernelGGL(k_clock_add, dim3(1), dim3(64), 0, b->stream, b->kb.clock,
+                       (int64_t)p.sample_T);
+    r = bhip(b, hipGetLastError(), "clock launch");
+  }
+  hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (r) {
+    if (g) (void)hipGraphDestroy(g);
+    return r;
+  }
+  r = bhip(b, e, "end capture");
+  if (r) return r;
+  r = bhip(b, hipGraphInstantiate(&b->graph[par], g, nullptr, nullptr, 0), "instantiate");
+  (void)hipGraphDestroy(g);
+  return r;
+}
+
+void drop_graphs(pm_batch* b) {
+  for (auto& g : b->graph)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int pm_batch_create(int device, void* stream, pm_batch** out) {
+  if (!out) return PM_EINVAL;
+  *out = nullptr;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) return (int)e;
+  if (device < 0 || device >= n) return PM_EINVAL;
+  e = hipSetDevice(device);
+  if (e != hipSuccess) return (int)e;
+  pm_batch* b = new pm_batch();
+  b->device = device;
+  b->stream = (hipStream_t)stream;
+  *out = b;
+  return 0;
+}
+
+int pm_batch_destroy(pm_batch* b) {
+  if (!b) return PM_EINVAL;
+  (void)hipSetDevice(b->device);
+  drop_graphs(b);
+  for (auto& v : b->spans)
+    for (auto& ab : v) {
+      (void)hipEventDestroy(ab.first);
+      (void)hipEventDestroy(ab.second);
+    }
+  delete b;
+  return 0;
+}
+
+const char* pm_batch_last_error(pm_batch* b) { return b ? b->err.c_str() : "null handle"; }
+
+int pm_batch_workspace_bytes(const pm_batch_plan* p, int64_t* bytes) {
+  if (!p || !bytes || p->n_design < 1 || !p->design || p->chunk < 2) return PM_EINVAL;
+  std::vector<KWork> w;
+  build_work(*p, w);
+  int64_t a, c, t;
+  layout(*p, &w, &a, &c, &t, (int64_t)w.size());
+  *bytes = t;
+  return 0;
+}
+
+int pm_batch_configure(pm_batch* b, const pm_batch_plan* p) {
+  if (!b || !p) return PM_EINVAL;
+  (void)hipSetDevice(b->device);
+  b->configured = false;
+  drop_graphs(b);
+  int r = validate(b, *p);
+  if (r) return r;
+  std::vector<KWork> work;
+  build_work(*p, work);
+  int64_t des_off, work_off, total;
+  layout(*p, &work, &des_off, &work_off, &total, (int64_t)work.size());
+  if (!p->work || p->work_bytes < total)
+    return bfail(b, PM_EINVAL, "workspace of %lld bytes required", (long long)total);
+  b->plan = *p;
+  b->des.assign(p->design, p->design + p->n_design);
+  b->plan.design = b->des.data();
+  b->work = std::move(work);
+  char* ws = (char*)p->work;
+  KBatch& k = b->kb;
+  k.clock = (int64_t*)ws;
+  k.des = (const pm_batch_design*)(ws + des_off);
+  k.work = (const KWork*)(ws + work_off);
+  k.n_work = (int)b->work.size();
+  k.ncols = p->ncols;
+  k.sample_T = p->sample_T;
+  k.idx = 1.0 / p->dx;
+  k.idy = 1.0 / p->dy;
+  k.tab_e = p->src_e_table;
+  k.tab_h = p->src_h_table;
+  k.table_step0 = p->table_step0;
+  k.X = p->X;
+  b->dft = KDft{p->dft.col0, p->dft.col1, p->dft.f0, p->dft.f1, p->dft.acc_re,
+                p->dft.acc_im, p->dft.cmp_re, p->dft.cmp_im};
+  r = bhip(b, hipMemcpyAsync((void*)k.des, b->des.data(),
+                             sizeof(pm_batch_design) * b->des.size(), hipMemcpyHostToDevice,
+                             b->stream), "design table upload");
+  if (!r)
+    r = bhip(b, hipMemcpyAsync((void*)k.work, b->work.data(), sizeof(KWork) * b->work.size(),
+                               hipMemcpyHostToDevice, b->stream), "tile list upload");
+  // the uploads read pageable host memory owned by this object: finish them
+  if (!r) r = bhip(b, hipStreamSynchronize(b->stream), "upload sync");
+  if (r) return r;
+  b->step = p->table_step0;
+  b->parity = 0;
+  b->pend0 = b->step;
+  b->pend_n = 0;
+  b->configured = true;
+  return 0;
+}
+
+int pm_batch_run(pm_batch* b, int32_t nsteps, uint32_t flags) {
+  if (!b) return PM_EINVAL;
+  if (!b->configured) return bfail(b, PM_ENOPLAN, "pm_batch_run before pm_batch_configure");
+  if (nsteps < 0) return bfail(b, PM_EINVAL, "nsteps < 0");
+  const pm_batch_plan& p = b->plan;
+  if (b->step + nsteps > p.table_step0 + p.table_len)
+    return bfail(b, PM_ERANGE, "source/phasor tables end at step %lld",
+                 (long long)(p.table_step0 + p.table_len));
+  (void)hipSetDevice(b->device);
+  const int T = p.sample_T;
+  const bool graphs = (flags & PM_RUN_GRAPH) && (T % 2 == 0);
+  if ((flags & PM_RUN_GRAPH) && !graphs)
+    return bfail(b, PM_EINVAL, "graph replay needs an even sample_T");
+  if (graphs && b->stream == nullptr)
+    return bfail(b, PM_EINVAL, "graph replay needs a non-default stream");
+  int64_t left = nsteps;
+  bool clock_ok = false;
+  while (left > 0) {
+    if (graphs && b->step % T == 0 && b->pend_n == 0 && left >= T) {
+      int r = 0;
+      if (!b->graph[b->parity]) r = capture(b, b->parity);
+      if (r) return r;
+      if (!clock_ok) {
+        hipLaunchKernelGGL(k_clock_set, dim3(1), dim3(64), 0, b->stream, b->kb.clock, b->step);
+        r = bhip(b, hipGetLastError(), "clock launch");
+        if (r) return r;
+        clock_ok = true;
+      }
+      {
+        BSpan sp(b, 2);
+        r = bhip(b, hipGraphLaunch(b->graph[b->parity], b->stream), "graph launch");
+      }
+      if (r) return r;
+      b->step += T;
+      b->pend0 = b->step;
+      left -= T;
+      continue;
+    }
+    clock_ok = false;
+    int r;
+    {
+      BSpan sp(b, 0);
+      r = step_once(b, b->step, 0, 0, b->parity);
+    }
+    if (r) return r;
+    if (b->pend_n == 0) b->pend0 = b->step;
+    ++b->pend_n;
+    b->parity ^= 1;
+    ++b->step;
+    --left;
+    if (b->step % T == 0) {
+      BSpan sp(b, 1);
+      r = flush_block(b, b->pend0, b->pend_n, nullptr);
+      if (r) return r;
+      b->pend_n = 0;
+      b->pend0 = b->step;
+    }
+  }
+  return 0;
+}
+
+int pm_batch_flush(pm_batch* b) {
+  if (!b) return PM_EINVAL;
+  if (!b->configured) return bfail(b, PM_ENOPLAN, "pm_batch_flush before configure");
+  (void)hipSetDevice(b->device);
+  int r = flush_block(b, b->pend0, b->pend_n, nullptr);
+  if (r) return r;
+  b->pend_n = 0;
+  b->pend0 = b->step;
+  return 0;
+}
+
+int pm_batch_parity(pm_batch* b) { return b ? b->parity : PM_EINVAL; }
+int64_t pm_batch_steps(pm_batch* b) { return b ? b->step - b->plan.table_step0 : PM_EINVAL; }
+
+int pm_batch_sync(pm_batch* b) {
+  if (!b) return PM_EINVAL;
+  (void)hipSetDevice(b->device);
+  return bhip(b, hipStreamSynchronize(b->stream), "sync");
+}
+
+int pm_batch_set_timing(pm_batch* b, int enable) {
+  if (!b) return PM_EINVAL;
+  b->timing = enable != 0;
+  for (auto& v : b->spans) {
+    for (auto& ab : v) {
+      (void)hipEventDestroy(ab.first);
+      (void)hipEventDestroy(ab.second);
+    }
+    v.clear();
+  }
+  return 0;
+}
+
+int pm_batch_get_timing(pm_batch* b, int which, double* avg_ms, int64_t* count) {
+  if (!b || which < 0 || which > 2 || !avg_ms || !count) return PM_EINVAL;
+  (void)hipSetDevice(b->device);
+  double tot = 0;
+  int64_t n = 0;
+  for (auto& ab : b->spans[which]) {
+    float ms = 0;
+    if (hipEventSynchronize(ab.second) != hipSuccess) continue;
+    if (hipEventElapsedTime(&ms, ab.first, ab.second) == hipSuccess) {
+      tot += ms;
+      ++n;
+    }
+  }
+  *avg_ms = n ? tot / n : 0.0;
+  *count = n;
+  return 0;
+}
+
+}  // extern "C"

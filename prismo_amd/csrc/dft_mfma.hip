@@ -59,8 +59,12 @@ constexpr int kDftWaves = 4;
 template <typename T, int FT>
 __global__ __launch_bounds__(64 * kDftWaves) void k_dft_mfma(
     const KDft d, const T* __restrict__ X, int ncols, const T* __restrict__ Wre,
-    const T* __restrict__ Wim, int64_t table_len, int64_t wcol0, int rows) {
+    const T* __restrict__ Wim, int64_t table_len, int64_t wcol0, int rows,
+    const int64_t* __restrict__ wclock) {
   using v4 = typename Mfma<T>::v4;
+  // in a captured step loop the block's first step is read from the device
+  // clock (wcol0 then holds the table's first step, subtracted here)
+  if (wclock != nullptr) wcol0 = *wclock - wcol0;
   const int np = d.col1 - d.col0;
   const int nf = d.f1 - d.f0;
   const int lane = threadIdx.x & 63;
@@ -123,7 +127,7 @@ __global__ __launch_bounds__(64 * kDftWaves) void k_dft_mfma(
 template <typename T>
 static void launch_t(const KDft& d, const void* X, int ncols, const void* W_re,
                      const void* W_im, int64_t table_len, int64_t wcol0, int rows,
-                     hipStream_t s) {
+                     const int64_t* wclock, hipStream_t s) {
   const int np = d.col1 - d.col0, nf = d.f1 - d.f0;
   const int strips = (np + 15) / 16;
   dim3 grid((strips + kDftWaves - 1) / kDftWaves);
@@ -131,23 +135,24 @@ static void launch_t(const KDft& d, const void* X, int ncols, const void* W_re,
   // frequency groups of 64 (FT = 4) unless the group is small
   if (nf > 32)
     hipLaunchKernelGGL((k_dft_mfma<T, 4>), grid, block, 0, s, d, (const T*)X, ncols,
-                       (const T*)W_re, (const T*)W_im, table_len, wcol0, rows);
+                       (const T*)W_re, (const T*)W_im, table_len, wcol0, rows, wclock);
   else if (nf > 16)
     hipLaunchKernelGGL((k_dft_mfma<T, 2>), grid, block, 0, s, d, (const T*)X, ncols,
-                       (const T*)W_re, (const T*)W_im, table_len, wcol0, rows);
+                       (const T*)W_re, (const T*)W_im, table_len, wcol0, rows, wclock);
   else
     hipLaunchKernelGGL((k_dft_mfma<T, 1>), grid, block, 0, s, d, (const T*)X, ncols,
-                       (const T*)W_re, (const T*)W_im, table_len, wcol0, rows);
+                       (const T*)W_re, (const T*)W_im, table_len, wcol0, rows, wclock);
 }
 
 void launch_dft(const KDft& d, const void* X, int ncols, const void* W_re, const void* W_im,
-                int64_t table_len, int64_t wcol0, int rows, int precision, hipStream_t s) {
+                int64_t table_len, int64_t wcol0, int rows, int precision, hipStream_t s,
+                const int64_t* wclock) {
   const int np = d.col1 - d.col0, nf = d.f1 - d.f0;
   if (np <= 0 || nf <= 0 || rows <= 0) return;
   if (precision == PM_FP64)
-    launch_t<double>(d, X, ncols, W_re, W_im, table_len, wcol0, rows, s);
+    launch_t<double>(d, X, ncols, W_re, W_im, table_len, wcol0, rows, wclock, s);
   else
-    launch_t<float>(d, X, ncols, W_re, W_im, table_len, wcol0, rows, s);
+    launch_t<float>(d, X, ncols, W_re, W_im, table_len, wcol0, rows, wclock, s);
 }
 
 }  // namespace pm

@@ -136,8 +136,8 @@ int launch_yee_fused_f64(const KGrid& g, int ck, const FusedBufs& fb, int vec, i
 bool fused_supported(int precision, int vec, int w, int ty);
 void launch_sources(const KAux& a, int64_t step, hipStream_t s);
 void launch_sample(const KAux& a, int64_t step, hipStream_t s);
-void launch_dft(const KDft& d, const void* X, int ncols, const void* W_re,
-                const void* W_im, int64_t table_len, int64_t wcol0, int rows,
-                int precision, hipStream_t s);
+void launch_dft(const KDft& d, const void* X, int ncols, const void* W_re, const void* W_im,
+                int64_t table_len, int64_t wcol0, int rows, int precision, hipStream_t s,
+                const int64_t* wclock = nullptr);
 
 }  // namespace pm

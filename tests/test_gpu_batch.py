@@ -1,0 +1,124 @@
+"""GPU: batched 2-D TM stepping (include/pmbatch.h, prismo_amd/core/batch.py)
+against the O1 oracle, design by design.
+
+Designs of uneven size (strip and chunk tails, several tiles per design)
+with CPML, a PlaneWaveSource line (Ez and Hy) and a FieldMonitor port line
+(Ez, Hy) with spectra.  FP64: fields and spectra within 1e-10 relative L2 of
+O1; FP32 fields with FP16 Cb: within 1e-5 of O1 fed the same dequantised
+Cb.  Graph replay of the step loop gives the same bits as eager launches.
+"""
+
+import numpy as np
+import pytest
+
+from tests.conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+D = 20e-9
+SIZES = [(70, 40), (130, 25), (45, 90)]
+FREQS = [187e12, 193e12, 200e12, 205e12]
+
+
+def _design(nxp, nyp, t, dtype, seed):
+    import prismo_amd as pa
+
+    rng = np.random.default_rng(seed)
+    s = 1 - 1e-9
+    g = pa.YeeGrid(pa.GridSpec(size=(nxp * D * s, nyp * D * s, 0.0), resolution=1 / D,
+                               boundary_layers=t))
+    nx, ny, _ = g.dimensions
+    eps = np.full((nx, ny, 1), 2.085)
+    yc = ny // 2 + int(rng.integers(-3, 4))
+    eps[:, yc - 5:yc + 6] = 12.1
+    eps *= rng.uniform(1.0, 1.05, size=eps.shape)
+    sim = pa.Simulation(size=(nxp * D * s, nyp * D * s, 0.0), resolution=1 / D, pml_layers=t,
+                        courant_factor=0.99, scheme="yee", pml=pa.PMLParams(t),
+                        material_arrays={"eps_rel": eps}, dtype=dtype)
+    y0 = (yc - t) * D
+    sim.add_source(pa.PlaneWaveSource(center=(6 * D, y0, 0), size=(0, 14 * D, 0), direction="+x",
+                                      polarization="z", frequency=193e12, pulse_width=5e-15))
+    mon = pa.FieldMonitor(center=((nxp - 6) * D, y0, 0), size=(0, 16 * D, 0),
+                          components=["Ez", "Hy"], frequencies=FREQS, time_domain=False)
+    sim.add_monitor(mon)
+    return sim, eps
+
+
+def _oracle(sim, eps, t, cb):
+    from oracle import compat as O0
+    from oracle.yee import YeeOracle
+    from oracle.compat import RegionDFT
+
+    nx, ny, _ = sim.grid.dimensions
+    _, Cb, _, Db = O0.coefficients(eps, np.ones_like(eps), np.zeros_like(eps),
+                                   np.zeros_like(eps), sim.dt)
+    o1 = YeeOracle(nx, ny, 1, sim.grid.spacing, sim.dt, {"Ez": cb}, float(Db.flat[0]),
+                   pml=(t, t, t, t, 0, 0))
+    mon = sim.monitors[0]
+    ref = RegionDFT({c: mon._monitor_region[c] for c in mon.components}, mon.frequencies)
+    return o1, ref
+
+
+def _drive(o1, ref, sim, n):
+    tt = 0.0
+    for _ in range(n):
+        o1.step()
+        tt += sim.dt
+        for s in sim.sources:
+            s.update_fields(o1.F, tt, sim.dt)
+        ref.update(o1.F, tt, sim.dt)
+
+
+def _run(dtype, storage, steps, graph, T=32, t=8):
+    from prismo_amd.core.batch import BatchEngine
+
+    built = [_design(nxp, nyp, t, dtype, 100 + k) for k, (nxp, nyp) in enumerate(SIZES)]
+    sims = [b[0] for b in built]
+    eng = BatchEngine(sims, dtype=dtype, coefficient_storage=storage, chunk=32, sample_T=T)
+    eng.run_steps(steps, graph=graph)
+    return eng, built
+
+
+@pytest.mark.parametrize("steps,graph", [(150, False), (151, False), (150, True)])
+def test_batch_fp64_matches_oracle(mi355x, steps, graph):
+    eng, built = _run(np.float64, "auto", steps, graph)
+    for d, (sim, eps) in enumerate(built):
+        o1, ref = _oracle(sim, eps, 8, eng.cb_host[d])
+        _drive(o1, ref, sim, steps)
+        got = eng.fields(d)
+        for c in ("Ez", "Hx", "Hy"):
+            assert rel_l2(got[c], o1.F[c]) <= 1e-10, (d, c)
+        mon = sim.monitors[0]
+        for c in mon.components:
+            for f in mon.frequencies:
+                assert rel_l2(mon.get_frequency_data(c, f), ref.data[c][f]) <= 1e-10, (d, c, f)
+
+
+def test_batch_fp32_fp16_matches_oracle(mi355x):
+    steps = 240
+    eng, built = _run(np.float32, "fp16", steps, True)
+    for d, (sim, eps) in enumerate(built):
+        o1, ref = _oracle(sim, eps, 8, eng.cb_host[d])
+        _drive(o1, ref, sim, steps)
+        got = eng.fields(d)
+        all_got = np.concatenate([got[c].ravel() for c in ("Ez", "Hx", "Hy")])
+        all_ref = np.concatenate([o1.F[c].ravel() for c in ("Ez", "Hx", "Hy")])
+        assert rel_l2(all_got, all_ref) <= 1e-5, d
+        mon = sim.monitors[0]
+        spec = np.stack([mon.get_frequency_data(c, f) for c in mon.components for f in FREQS])
+        sref = np.stack([ref.data[c][f] for c in mon.components for f in FREQS])
+        assert rel_l2(spec, sref) <= 1e-5, d
+
+
+def test_batch_graph_equals_eager(mi355x):
+    out = []
+    for graph in (False, True):
+        eng, built = _run(np.float32, "auto", 200, graph)
+        f = [eng.fields(d) for d in range(len(built))]
+        s = [np.stack([b[0].monitors[0].get_frequency_data(c, fr) for c in ("Ez", "Hy")
+                       for fr in FREQS]) for b in built]
+        out.append((f, s))
+    (fa, sa), (fb, sb) = out
+    for d in range(len(SIZES)):
+        for c in ("Ez", "Hx", "Hy"):
+            assert np.array_equal(fa[d][c], fb[d][c]), (d, c)
+        assert np.array_equal(sa[d], sb[d]), d

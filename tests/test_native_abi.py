@@ -12,15 +12,19 @@ from prismo_amd import _native as N
 
 ROOT = Path(__file__).resolve().parents[1]
 HEADER = ROOT / "include" / "pmfdtd.h"
+HEADERS = sorted((ROOT / "include").glob("*.h"))
 
 
 def declared_functions():
-    text = HEADER.read_text()
-    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\*?\s+\**(pm_\w+)\s*\(", text, re.M)))
+    names = set()
+    for h in HEADERS:
+        names |= set(re.findall(r"^\s*(?:const\s+)?\w+\*?\s+\**(pm_\w+)\s*\(", h.read_text(),
+                                re.M))
+    return sorted(names)
 
 
 def test_header_declares_what_binding_binds():
-    assert set(declared_functions()) == set(N.SIGNATURES)
+    assert set(declared_functions()) == set(N.SIGNATURES) | set(N.BATCH_SIGNATURES)
 
 
 def test_library_loads_and_exports_all_symbols():
@@ -41,20 +45,23 @@ PROBE = r"""
 #include <stddef.h>
 #include <stdio.h>
 #include "pmfdtd.h"
+#include "pmbatch.h"
 #define P(T, f) printf(#T "." #f " %zu\n", offsetof(T, f));
 int main(void) {
-  printf("pm_plan %zu\npm_source %zu\npm_sample_box %zu\npm_dft_group %zu\npm_ade_box %zu\n",
-         sizeof(pm_plan), sizeof(pm_source), sizeof(pm_sample_box), sizeof(pm_dft_group),
-         sizeof(pm_ade_box));
   %FIELDS%
   return 0;
 }
 """
 
 
+STRUCTS = (N.pm_plan, N.pm_source, N.pm_sample_box, N.pm_dft_group, N.pm_ade_box,
+           N.pm_batch_line, N.pm_batch_design, N.pm_batch_plan)
+
+
 def test_struct_layout_matches_header(tmp_path):
     lines = []
-    for cls in (N.pm_plan, N.pm_source, N.pm_sample_box, N.pm_dft_group, N.pm_ade_box):
+    for cls in STRUCTS:
+        lines.append(f'printf("{cls.__name__} %zu\\n", sizeof({cls.__name__}));')
         for fname, _ in cls._fields_:
             lines.append(f"P({cls.__name__}, {fname})")
     src = tmp_path / "probe.c"
@@ -63,7 +70,7 @@ def test_struct_layout_matches_header(tmp_path):
     subprocess.run(["gcc", "-std=c11", f"-I{HEADER.parent}", str(src), "-o", str(exe)], check=True)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
     got = dict(l.rsplit(" ", 1) for l in out if l.strip())
-    for cls in (N.pm_plan, N.pm_source, N.pm_sample_box, N.pm_dft_group, N.pm_ade_box):
+    for cls in STRUCTS:
         assert int(got[cls.__name__]) == C.sizeof(cls), cls.__name__
         for fname, _ in cls._fields_:
             assert int(got[f"{cls.__name__}.{fname}"]) == getattr(cls, fname).offset, (cls, fname)
