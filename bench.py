@@ -20,6 +20,14 @@ by the rocprofv3 FETCH_SIZE / WRITE_SIZE counters (scaled by a copy-kernel
 calibration), read from the committed profile summary for this workload
 when it is present.
 
+The same line carries ``sweep``: 256 seeded directional couplers (2-D TM,
+200x120 .. 800x400 cells + 16-cell CPML, 4000 steps, 64-frequency ports)
+advanced together in FP32 with FP16 Cb by the batched kernel, timed end to
+end (build, run, |S21| analysis) and on the GPU, beside sequential plain
+FP64 runs of an evenly spaced sample of the same designs through
+Simulation.run; ``max_abs_s21_err`` is the largest |S21| difference
+between the two over that sample and all 64 frequencies.
+
 ``--copy-probe`` runs only a 256 MiB device copy (the counter calibration
 kernel); ``--launch vec,ty,kchunk,order`` fixes the fused launch shape
 instead of autotuning.
@@ -120,6 +128,84 @@ def cpu_baseline(n=256, steps=3):
     }
 
 
+SWEEP_ALG_BYTES_PER_CELL = 26.0  # 2-D TM: Ez, Hx, Hy FP32 read + written, FP16 Cb
+SWEEP_ALG_BYTES_PER_PML_CELL_AXIS = 16.0  # two FP32 psi read + written
+
+
+def sweep_bench(n_designs=256, steps=4000, seq_sample=12):
+    """Batched 256-coupler sweep (FP32 fields, FP16 Cb, one launch per step
+    over every design, graph-replayed step loop) against sequential plain
+    FP64 runs of an evenly spaced sample of the same designs."""
+    import numpy as np
+    import torch
+
+    from prismo_amd import workloads as W
+    from prismo_amd.core.batch import BatchEngine
+
+    designs = W.coupler_designs(n_designs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sims = [W.coupler_sim(p, backend=W.host_backend()) for p in designs]
+    eng = BatchEngine(sims, dtype=np.float32, coefficient_storage="fp16", chunk=16,
+                      sample_T=128)
+    t1 = time.perf_counter()
+    eng.run_steps(steps, graph=True, timing=True)
+    t2 = time.perf_counter()
+    res = [W.coupler_analyze(s) for s in sims]
+    t3 = time.perf_counter()
+    g_ms, g_n = eng.timing(2)
+    e_ms, e_n = eng.timing(0)
+    f_ms, f_n = eng.timing(1)
+    gpu_ms = g_ms * g_n + e_ms * e_n + f_ms * f_n
+    cells = 0
+    slab = 0
+    for s in sims:
+        nx, ny, _ = s.grid.dimensions
+        cells += nx * ny
+        t = W.COUPLER_PML
+        slab += 2 * t * ny + 2 * t * nx
+    alg = SWEEP_ALG_BYTES_PER_CELL * cells + SWEEP_ALG_BYTES_PER_PML_CELL_AXIS * slab
+    step_ms = gpu_ms / steps
+    print(f"[bench] sweep: build {t1 - t0:.2f} s, run {t2 - t1:.2f} s (GPU {gpu_ms / 1e3:.2f} s, "
+          f"{g_n} graph replays), analyse {t3 - t2:.2f} s", file=sys.stderr, flush=True)
+    idx = np.unique(np.linspace(0, n_designs - 1, seq_sample).round().astype(int))
+    seq_t = 0.0
+    err21 = err31 = 0.0
+    for k in idx:
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        sim = W.coupler_sim(designs[k], dtype=np.float64)
+        sim.run_steps(steps)
+        r = W.coupler_analyze(sim)
+        torch.cuda.synchronize()
+        seq_t += time.perf_counter() - a
+        err21 = max(err21, float(np.max(np.abs(res[k]["s21"] - r["s21"]))))
+        err31 = max(err31, float(np.max(np.abs(res[k]["s31"] - r["s31"]))))
+        del sim
+    seq_rate = len(idx) / seq_t
+    rate = n_designs / (t3 - t0)
+    print(f"[bench] sweep: FP64 sequential {len(idx)} designs in {seq_t:.2f} s", file=sys.stderr,
+          flush=True)
+    return {
+        "designs": n_designs,
+        "steps": steps,
+        "designs_per_s": round(rate, 3),
+        "designs_per_s_gpu_only": round(n_designs / (gpu_ms / 1e3), 3),
+        "fp64_sequential_designs_per_s": round(seq_rate, 4),
+        "fp64_sample": int(len(idx)),
+        "speedup": round(rate / seq_rate, 2),
+        "max_abs_s21_err": err21,
+        "max_abs_s31_err": err31,
+        "cells_per_step": cells,
+        "gcells_per_s": round(cells * steps / (gpu_ms / 1e3) / 1e9, 3),
+        "roofline": {"bound": "hbm", "kernel": "k_batch_tm (one launch per step, all designs)",
+                     "achieved": round(alg / (step_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(alg / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "alg_bytes_per_step": alg, "step_ms": round(step_ms, 5)},
+        "precision": "fp32 fields, fp16 Cb, fp32 compensated spectra; baseline fp64",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,6 +216,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--launch", type=str, default=None)
     ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--sweep-only", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only; runs on several GPUs are not supported "
@@ -146,6 +234,9 @@ def main():
     from prismo_amd.workloads import voxel_3d
 
     pa.set_backend("mi355x", 0)
+    if args.sweep_only:
+        print(json.dumps({"sweep": sweep_bench()}), flush=True)
+        return
     sim = voxel_3d(dtype=np.float32, coefficient_storage="fp16", n=args.n)
     eng = sim.solver.updater.engine_for(sim.fields)
     print(f"[bench] built {args.n}^3 FP32/FP16 voxel workload", file=sys.stderr, flush=True)
@@ -186,6 +277,9 @@ def main():
     print(f"[bench] {args.steps} steps in {el:.3f} s, fused {f_ms:.4f} ms x{nf}, "
           f"H {h_ms:.4f} ms x{nh}, E {e_ms:.4f} ms x{ne}, "
           f"finite={finite}", file=sys.stderr, flush=True)
+    del sim, eng
+    torch.cuda.empty_cache()
+    sweep = None if args.no_sweep else sweep_bench()
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] timing the CPU oracle sample", file=sys.stderr, flush=True)
@@ -223,6 +317,7 @@ def main():
             "kernel_ms": round(kern_ms, 5),
         },
         "cpu_baseline": cpu,
+        "sweep": sweep,
     }
     print(json.dumps(line), flush=True)
 

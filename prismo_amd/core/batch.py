@@ -38,7 +38,7 @@ def _torch():
 
 class BatchEngine:
     def __init__(self, sims, dtype=np.float32, coefficient_storage: str = "auto",
-                 chunk: int = 32, sample_T: int = 128, device: int = 0):
+                 chunk: int = 16, sample_T: int = 128, device: int = 0):
         torch = _torch()
         if not sims:
             raise ValueError("empty batch")
@@ -80,8 +80,11 @@ class BatchEngine:
         self._keep.append(t)
         return t
 
+    _FIELD_ARRAYS = ("ez", "hx", "hy", "psi_hx", "psi_hy")
+
     def _design(self, s, des, col):
-        torch = _torch()
+        """Geometry, coefficients, CPML and sample lines of one design; the
+        device arrays are placed afterwards in shared arenas."""
         up = s.solver.updater
         g = s.grid
         nx, ny, _ = g.dimensions
@@ -94,27 +97,17 @@ class BatchEngine:
         des.db = float(Db.flat[0])
         cb = component_average(np.asarray(up.Cb)[:, :, 0], "Ez", True)   # (nx-1, ny-1)
         scale = self.dt / EPS0
-        buf = {}
-        for name in ("ez", "hx", "hy", "psi_hx", "psi_hy"):
-            buf[name] = [self._alloc((ny, sy)), self._alloc((ny, sy))]
-            getattr(des, name)[0] = buf[name][0].data_ptr()
-            getattr(des, name)[1] = buf[name][1].data_ptr()
-        for name in ("psi_ex", "psi_ey"):
-            buf[name] = self._alloc((ny, sy))
-            setattr(des, name, buf[name].data_ptr())
         if self.coeff_kind == N.COEFF_CB_F16:
             q = (cb / scale).astype(np.float16)
-            cbt = self._alloc((ny, sy), torch.float16)
-            cbt[: ny - 1, : nx - 1] = torch.from_numpy(np.ascontiguousarray(q.T)).to(self.device)
+            cb_pad = np.zeros((ny, sy), np.float16)
+            cb_pad[: ny - 1, : nx - 1] = q.T
             des.cb_scale = scale
             self.cb_host.append(q.astype(np.float64) * scale)
         else:
-            cbt = self._alloc((ny, sy))
-            cbt[: ny - 1, : nx - 1] = torch.from_numpy(np.ascontiguousarray(cb.T)).to(
-                self.device, self.tdtype)
+            cb_pad = np.zeros((ny, sy), np.float64 if self.precision == N.FP64 else np.float32)
+            cb_pad[: ny - 1, : nx - 1] = cb.T
             des.cb_scale = 1.0
             self.cb_host.append(cb)
-        des.cb = cbt.data_ptr()
         # CPML: x then y profiles, six arrays each
         cp = up.cpml
         th = cp.thickness if cp is not None else (0, 0, 0, 0, 0, 0)
@@ -131,10 +124,8 @@ class BatchEngine:
                     continue
                 for k, key in enumerate(("be", "ce", "ike", "bh", "ch", "ikh")):
                     prof[base + k * n: base + (k + 1) * n] = pr[a][key]
-        pt = self._alloc((prof.size,))
-        pt.copy_(torch.from_numpy(prof).to(self.device, self.tdtype))
-        des.prof = pt.data_ptr()
-        self._bufs.append(buf)
+        self._host_cb.append(cb_pad)
+        self._host_prof.append(prof)
         # monitors -> sample lines
         nl = 0
         mons = []
@@ -176,10 +167,55 @@ class BatchEngine:
                     raise NotImplementedError("one Ez line and one Hy line per design")
         return e, h
 
+    def _place_arrays(self, arr):
+        """One zero-filled arena for every design's fields and CPML memory,
+        one upload each for the coefficients and the profiles."""
+        torch = _torch()
+        n = len(self.sims)
+        sizes = [arr[d].ny * arr[d].sy for d in range(n)]
+        per = 2 * len(self._FIELD_ARRAYS) + 2
+        arena = self._alloc((per * sum(sizes),))
+        cb_all = torch.from_numpy(np.concatenate([c.ravel() for c in self._host_cb]))
+        cbd = cb_all.to(self.device, torch.float16 if self.coeff_kind == N.COEFF_CB_F16
+                        else self.tdtype)
+        pr_all = torch.from_numpy(np.concatenate(self._host_prof)).to(self.device, self.tdtype)
+        self._keep += [cbd, pr_all]
+        off = 0
+        coff = 0
+        poff = 0
+        self._bufs = []
+        for d in range(n):
+            des = arr[d]
+            ny, sy, m = des.ny, des.sy, sizes[d]
+
+            def take():
+                nonlocal off
+                t = arena[off:off + m].view(ny, sy)
+                off += m
+                return t
+
+            buf = {}
+            for name in self._FIELD_ARRAYS:
+                buf[name] = [take(), take()]
+                getattr(des, name)[0] = buf[name][0].data_ptr()
+                getattr(des, name)[1] = buf[name][1].data_ptr()
+            for name in ("psi_ex", "psi_ey"):
+                buf[name] = take()
+                setattr(des, name, buf[name].data_ptr())
+            des.cb = cbd[coff:coff + m].data_ptr()
+            coff += m
+            np_ = self._host_prof[d].size
+            des.prof = pr_all[poff:poff + np_].data_ptr()
+            poff += np_
+            self._bufs.append(buf)
+        self._host_cb = []
+        self._host_prof = []
+
     def _build(self, nsteps: int):
         torch = _torch()
         n = len(self.sims)
-        self._bufs = []
+        self._host_cb = []
+        self._host_prof = []
         arr = (N.pm_batch_design * n)()
         times = []
         t = 0.0
@@ -215,6 +251,7 @@ class BatchEngine:
                         tab_h = got[3]
                 elif not np.array_equal(ref, got[3]):
                     raise NotImplementedError("every design of a batch shares its source waveform")
+        self._place_arrays(arr)
         zeros = np.zeros(max(nsteps, 1))
         tab_e = zeros if tab_e is None else tab_e
         tab_h = zeros if tab_h is None else tab_h

@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <cstdarg>
 #include <cstdio>
 #include <string>
@@ -49,20 +50,31 @@ struct KBatch {
   void* X;
 };
 
+// Loads of one row for the H update (fields, H-side CPML memory and the
+// y profile at nodes) and of one row for the E update (Cb, E-side CPML
+// memory, y profile at half nodes).  They are issued one iteration before
+// their use; the loop below alternates two register sets instead of copying
+// so no wait lands on the loop's back edge.
+template <typename T>
+struct RowH {
+  T ez, hx, hy, psx, psy, b, c, ik;
+};
+
 template <typename T, int CK>
-__device__ __forceinline__ T cb_at(const pm_batch_design& D, int64_t o) {
-  if constexpr (CK == PM_COEFF_CB_F16) {
-    return (T)D.cb_scale * (T)__half2float(((const __half*)D.cb)[o]);
-  } else {
-    return ((const T*)D.cb)[o];
-  }
-}
+struct RowE {
+  using CbT = typename std::conditional<CK == PM_COEFF_CB_F16, unsigned short, T>::type;
+  CbT cb;
+  T pex, pey, b, c, ik;
+};
 
 template <typename T, int CK>
 __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_t step_arg,
                                                           int use_clock, int s_rel, int par) {
   const int lane = threadIdx.x & 63;
-  const int item = blockIdx.x * kWaves + (int)(threadIdx.x >> 6);
+  // the wave index is uniform: make that visible so the tile and design
+  // descriptors are scalar loads
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int item = blockIdx.x * kWaves + wv;
   if (item >= k.n_work) return;  // whole wave
   const KWork w = k.work[item];
   const pm_batch_design& D = k.des[w.d];
@@ -73,20 +85,20 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
   const bool inx = i >= 0 && i < nx;
   const bool own = lane >= 1 && lane <= kStrip && inx;
   const int q = par, qn = par ^ 1;
-  const T* EzA = (const T*)D.ez[q];
-  const T* HxA = (const T*)D.hx[q];
-  const T* HyA = (const T*)D.hy[q];
-  T* EzB = (T*)D.ez[qn];
-  T* HxB = (T*)D.hx[qn];
-  T* HyB = (T*)D.hy[qn];
-  const T* psHxA = (const T*)D.psi_hx[q];
-  const T* psHyA = (const T*)D.psi_hy[q];
-  T* psHxB = (T*)D.psi_hx[qn];
-  T* psHyB = (T*)D.psi_hy[qn];
-  T* psEx = (T*)D.psi_ex;
-  T* psEy = (T*)D.psi_ey;
-  const T* prof = (const T*)D.prof;
-  const T* py = prof + 6 * (int64_t)nx;
+  const T* __restrict__ EzA = (const T*)D.ez[q];
+  const T* __restrict__ HxA = (const T*)D.hx[q];
+  const T* __restrict__ HyA = (const T*)D.hy[q];
+  T* __restrict__ EzB = (T*)D.ez[qn];
+  T* __restrict__ HxB = (T*)D.hx[qn];
+  T* __restrict__ HyB = (T*)D.hy[qn];
+  const T* __restrict__ psHxA = (const T*)D.psi_hx[q];
+  const T* __restrict__ psHyA = (const T*)D.psi_hy[q];
+  T* __restrict__ psHxB = (T*)D.psi_hx[qn];
+  T* __restrict__ psHyB = (T*)D.psi_hy[qn];
+  T* __restrict__ psEx = (T*)D.psi_ex;
+  T* __restrict__ psEy = (T*)D.psi_ey;
+  const T* __restrict__ prof = (const T*)D.prof;
+  const T* __restrict__ py = prof + 6 * (int64_t)nx;
 
   // x profiles of this lane (identity outside the slabs)
   const bool xpml = inx && (i < D.pml[0] || i >= nx - D.pml[1]);
@@ -105,51 +117,105 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
   const T se = (T)k.tab_e[trow];
   const T sh = (T)k.tab_h[trow];
   T* Xrow = (T*)k.X + (step % k.sample_T) * (int64_t)k.ncols;
+  // everything the row loop needs from the design table, read once: the
+  // loop's stores could alias the table as far as the compiler knows
+  const int pml_y0 = D.pml[2], pml_y1 = ny - D.pml[3];
   const bool srcE = i == D.src_e[0];
   const bool srcH = i == D.src_h[0];
+  const int se0 = D.src_e[1], se1 = D.src_e[2];
+  const int sh0 = D.src_h[1], sh1 = D.src_h[2];
+  const T cbs = (T)D.cb_scale;
+  using CbT = typename RowE<T, CK>::CbT;
+  const CbT* __restrict__ cbp = (const CbT*)D.cb;
+  // sample lines at this lane's column (bit l: line l)
+  unsigned lmask = 0;
+  const int nl = D.n_lines;
+  for (int l = 0; l < nl; ++l)
+    if (D.line[l].i == i) lmask |= 1u << l;
+  if (!own) lmask = 0;
+  const bool cbx = inx && i < nx - 1;
+
+  auto load_h = [&](int j, RowH<T>& r) __attribute__((always_inline)) {
+    const int64_t o = (int64_t)j * sy + i;
+    const bool yp = j < pml_y0 || j >= pml_y1;
+    r.ez = 0; r.hx = 0; r.hy = 0; r.psx = 0; r.psy = 0;
+    if (inx) {
+      r.ez = EzA[o];
+      r.hx = HxA[o];
+      r.hy = HyA[o];
+      if (yp) r.psx = psHxA[o];
+    }
+    if (xpml) r.psy = psHyA[o];
+    r.b = 1; r.c = 0; r.ik = 1;
+    if (yp) {
+      r.b = py[3 * ny + j];
+      r.c = py[4 * ny + j];
+      r.ik = py[5 * ny + j];
+    }
+  };
+  auto load_e = [&](int jj, RowE<T, CK>& r) __attribute__((always_inline)) {
+    const int64_t o = (int64_t)jj * sy + i;
+    const bool yp = jj < pml_y0 || jj >= pml_y1;
+    const bool valid = cbx && jj < ny - 1;
+    r.cb = 0; r.pex = 0; r.pey = 0;
+    if (valid) {
+      r.cb = cbp[o];
+      if (xpml) r.pex = psEx[o];
+      if (yp) r.pey = psEy[o];
+    }
+    r.b = 1; r.c = 0; r.ik = 1;
+    if (yp) {
+      r.b = py[jj];
+      r.c = py[ny + jj];
+      r.ik = py[2 * ny + jj];
+    }
+  };
 
   const int r0 = w.r0, r1 = w.r1;
   const int jend = min(r1, ny - 1);
   T ezp = (r0 >= 1 && inx) ? EzA[(int64_t)(r0 - 1) * sy + i] : (T)0;
   T hxp = 0, hyp = 0;
-  for (int j = r0; j <= jend; ++j) {
+
+  // one row: H(n+1/2) at row j from `cur`, E(n+1) at row j-1 from `ecur`;
+  // loads row j+1 into `nxt` and the E row j into `enxt` first
+  auto body = [&](int j, const RowH<T>& cur, RowH<T>& nxt, const RowE<T, CK>& ecur,
+                  RowE<T, CK>& enxt) __attribute__((always_inline)) {
+    if (j < jend) load_h(j + 1, nxt);
+    if (j < jend) load_e(j, enxt);
     const int64_t o = (int64_t)j * sy + i;
-    const T ez = inx ? EzA[o] : (T)0;
-    const T hx = inx ? HxA[o] : (T)0;
-    const T hy = inx ? HyA[o] : (T)0;
-    const bool ypml = j < D.pml[2] || j >= ny - D.pml[3];
     const bool store_h = own && j < r1;
+    const bool yp = j < pml_y0 || j >= pml_y1;
 
     // Hx(i, j) -= Db * dEz/dy (backward difference, Ez(j = -1) = 0)
-    T dy = (ez - ezp) * idy;
-    if (ypml) {
-      const T b = py[3 * ny + j], c = py[4 * ny + j], ik = py[5 * ny + j];
-      T ps = inx ? psHxA[o] : (T)0;
-      ps = b * ps + c * dy;
+    T dy = (cur.ez - ezp) * idy;
+    if (yp) {
+      const T ps = cur.b * cur.psx + cur.c * dy;
       if (store_h) psHxB[o] = ps;
-      dy = dy * ik + ps;
+      dy = dy * cur.ik + ps;
     }
-    const T hxn = (inx && i < nx - 1) ? hx - db * dy : (T)0;
+    const T hxn = (inx && i < nx - 1) ? cur.hx - db * dy : (T)0;
 
     // Hy(i, j) += Db * dEz/dx, Ez(i-1) from the lane below
-    const T ezl = __shfl_up(ez, 1);
-    T dx = (ez - ezl) * idx;
+    const T ezl = __shfl_up(cur.ez, 1);
+    T dx = (cur.ez - ezl) * idx;
     if (xpml) {
-      T ps = psHyA[o];
-      ps = bhx * ps + chx * dx;
+      const T ps = bhx * cur.psy + chx * dx;
       if (store_h) psHyB[o] = ps;
       dx = dx * ikhx + ps;
     }
-    const T hyn = (inx && j < ny - 1) ? hy + db * dx : (T)0;
+    const T hyn = (inx && j < ny - 1) ? cur.hy + db * dx : (T)0;
 
     if (store_h) {
       HxB[o] = hxn;
-      const T hys = (srcH && j >= D.src_h[1] && j < D.src_h[2]) ? hyn + sh : hyn;
+      const T hys = (srcH && j >= sh0 && j < sh1) ? hyn + sh : hyn;
       HyB[o] = hys;
-      for (int l = 0; l < D.n_lines; ++l) {
-        const pm_batch_line& L = D.line[l];
-        if (L.comp != 0 && i == L.i && j >= L.j0 && j < L.j1)
-          Xrow[L.col + (j - L.j0)] = L.comp == 2 ? hys : hxn;
+      if (lmask) {
+        for (int l = 0; l < nl; ++l) {
+          if (!((lmask >> l) & 1u)) continue;
+          const pm_batch_line L = D.line[l];
+          if (L.comp != 0 && j >= L.j0 && j < L.j1)
+            Xrow[L.col + (j - L.j0)] = L.comp == 2 ? hys : hxn;
+        }
       }
     }
 
@@ -158,39 +224,57 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
       const int jj = j - 1;
       const int64_t oe = o - sy;
       const T hyr = __shfl_down(hyp, 1);
-      const bool valid = inx && i < nx - 1 && jj < ny - 1;
+      const bool valid = cbx && jj < ny - 1;
       T tx = (hyr - hyp) * idx;
       if (xpml && valid) {
-        T ps = psEx[oe];
-        ps = bex * ps + cex * tx;
+        const T ps = bex * ecur.pex + cex * tx;
         if (own) psEx[oe] = ps;
         tx = tx * ikex + ps;
       }
       T ty = (hxn - hxp) * idy;
-      if ((jj < D.pml[2] || jj >= ny - D.pml[3]) && valid) {
-        const T b = py[jj], c = py[ny + jj], ik = py[2 * ny + jj];
-        T ps = psEy[oe];
-        ps = b * ps + c * ty;
+      if ((jj < pml_y0 || jj >= pml_y1) && valid) {
+        const T ps = ecur.b * ecur.pey + ecur.c * ty;
         if (own) psEy[oe] = ps;
-        ty = ty * ik + ps;
+        ty = ty * ecur.ik + ps;
+      }
+      T cb;
+      if constexpr (CK == PM_COEFF_CB_F16) {
+        __half_raw hr;
+        hr.x = ecur.cb;
+        cb = cbs * (T)__half2float(__half(hr));
+      } else {
+        cb = ecur.cb;
       }
       T ezn = (T)0;
       if (valid) {
-        ezn = ezp + cb_at<T, CK>(D, oe) * (tx - ty);
-        if (srcE && jj >= D.src_e[1] && jj < D.src_e[2]) ezn += se;
+        ezn = ezp + cb * (tx - ty);
+        if (srcE && jj >= se0 && jj < se1) ezn += se;
       }
       if (own) {
         EzB[oe] = ezn;
-        for (int l = 0; l < D.n_lines; ++l) {
-          const pm_batch_line& L = D.line[l];
-          if (L.comp == 0 && i == L.i && jj >= L.j0 && jj < L.j1)
-            Xrow[L.col + (jj - L.j0)] = ezn;
+        if (lmask) {
+          for (int l = 0; l < nl; ++l) {
+            if (!((lmask >> l) & 1u)) continue;
+            const pm_batch_line L = D.line[l];
+            if (L.comp == 0 && jj >= L.j0 && jj < L.j1) Xrow[L.col + (jj - L.j0)] = ezn;
+          }
         }
       }
     }
-    ezp = ez;
+    ezp = cur.ez;
     hxp = hxn;
     hyp = hyn;
+  };
+
+  RowH<T> ha, hb;
+  RowE<T, CK> ea, eb;
+  load_h(r0, ha);
+  ea = RowE<T, CK>{};
+  for (int j = r0;;) {
+    body(j, ha, hb, ea, eb);
+    if (++j > jend) break;
+    body(j, hb, ha, eb, ea);
+    if (++j > jend) break;
   }
 }
 

@@ -9,6 +9,13 @@
   {1.0, 2.085, 12.1} with numpy.random.default_rng(20261202), Ez dipole at
   the centre with a 193 THz Gaussian pulse (10 fs, delay 30 fs), Courant
   0.99 of the 3-D limit, 1000 steps.
+* ``coupler_2d`` -- directional couplers for sweeps: two 450 nm Si strips
+  (eps 12.1) in SiO2 (eps 2.085) at 20 nm, gap g ~ U(100, 400) nm and
+  coupling length L ~ U(2, 12) um from the seeded generator, grid
+  nx = 200 + 600 (L - 2 um) / 10 um, ny = 120 + 280 (g - 100 nm) / 300 nm
+  physical cells (+16-cell CPML each side), Gaussian pulse at 193 THz on the
+  through strip, 64-frequency (180-210 THz) Ez/Hy port lines at the input,
+  the through and the cross output; 4000 steps.
 """
 
 from __future__ import annotations
@@ -68,3 +75,101 @@ def voxel_3d(dtype=np.float32, coefficient_storage="fp16", n=256, pml=12, backen
     sim.add_source(pa.ElectricDipole(position=(c, c, c), polarization="z", frequency=193e12,
                                      pulse_width=10e-15))
     return sim
+
+
+# ------------------------------------------------------------------ couplers
+COUPLER_FREQS = np.linspace(180e12, 210e12, 64)
+COUPLER_PML = 16
+COUPLER_STEPS = 4000
+
+
+def coupler_designs(n: int = 256, seed: int = SEED, width: bool = False):
+    """Seeded design parameters: gap, length (and, with ``width``, strip width
+    U(400, 500) nm and cladding eps U(2.0, 2.2))."""
+    rng = np.random.default_rng(seed)
+    gap = rng.uniform(100e-9, 400e-9, n)
+    length = rng.uniform(2e-6, 12e-6, n)
+    out = []
+    for k in range(n):
+        d = {"gap": float(gap[k]), "length": float(length[k]), "width": 450e-9,
+             "clad_eps": 2.085}
+        out.append(d)
+    if width:
+        w = rng.uniform(400e-9, 500e-9, n)
+        ce = rng.uniform(2.0, 2.2, n)
+        for k in range(n):
+            out[k]["width"] = float(w[k])
+            out[k]["clad_eps"] = float(ce[k])
+    return out
+
+
+def coupler_cells(p):
+    nx = 200 + int(round(600 * (p["length"] - 2e-6) / 10e-6))
+    ny = 120 + int(round(280 * (p["gap"] - 100e-9) / 300e-9))
+    return nx, ny
+
+
+def host_backend():
+    """A NumPy backend instance for Simulations that only describe a design
+    (batched runs): passing the instance, not the name, leaves the global
+    backend as it is."""
+    from .backends.numpy_backend import NumPyBackend
+
+    return NumPyBackend()
+
+
+def coupler_sim(p, dtype=np.float32, backend=None, coefficient_storage="auto",
+                freqs=COUPLER_FREQS):
+    """Build (not run) the Simulation of one coupler design."""
+    import prismo_amd as pa
+
+    t = COUPLER_PML
+    nxp, nyp = coupler_cells(p)
+    w, g = p["width"], p["gap"]
+    nx, ny = nxp + 2 * t, nyp + 2 * t
+    x = (np.arange(nx) - t) * D
+    y = (np.arange(ny) - t) * D
+    ymid = nyp * D / 2
+    y1 = ymid - (g + w) / 2
+    y2 = ymid + (g + w) / 2
+    eps = np.full((nx, ny, 1), p["clad_eps"])
+    eps[:, np.abs(y - y1) <= w / 2, 0] = 12.1
+    sel2 = np.abs(y - y2) <= w / 2
+    eps[np.ix_(x >= 1.5e-6, sel2, [0])] = 12.1
+    sim = pa.Simulation(size=(nxp * D * SHRINK, nyp * D * SHRINK, 0.0), resolution=1 / D,
+                        pml_layers=t, courant_factor=0.99, scheme="yee", dtype=dtype,
+                        backend=backend, pml=pa.PMLParams(thickness=t),
+                        material_arrays={"eps_rel": eps},
+                        coefficient_storage=coefficient_storage)
+    sim.add_source(pa.PlaneWaveSource(center=(0.6e-6, y1, 0.0), size=(0.0, w + 0.6e-6, 0.0),
+                                      direction="+x", polarization="z", frequency=193e12,
+                                      pulse_width=8e-15))
+    lo = max(y1 - w / 2 - 0.5e-6, 0.0)
+    hi = min(y2 + w / 2 + 0.5e-6, nyp * D * SHRINK)
+    xo = nxp * D - 0.6e-6
+    ports = {"in": (1.0e-6, lo, ymid), "thru": (xo, lo, ymid), "cross": (xo, ymid, hi)}
+    sim.ports = {}
+    for name, (px, a, b) in ports.items():
+        m = pa.FieldMonitor(center=(px, (a + b) / 2, 0.0), size=(0.0, b - a, 0.0),
+                            components=["Ez", "Hy"], frequencies=list(freqs),
+                            time_domain=False, name=name)
+        sim.add_monitor(m)
+        sim.ports[name] = m
+    return sim
+
+
+def coupler_analyze(sim, params=None) -> dict:
+    """|S21| (through) and |S31| (cross) from the port fluxes, normalised to
+    the input port's forward power."""
+    from .analysis.sparameters import SParameterAnalyzer, port_power
+
+    ports = sim.ports
+    freqs = ports["in"].frequencies
+    sa = SParameterAnalyzer(3, freqs)
+    p_in = port_power(ports["in"], "x", +1)
+    zero = np.zeros_like(p_in)
+    sa.add_port_data(0, 0, p_in, zero)
+    sa.add_port_data(1, 0, np.maximum(port_power(ports["thru"], "x", +1), 0.0), zero)
+    sa.add_port_data(2, 0, np.maximum(port_power(ports["cross"], "x", +1), 0.0), zero)
+    return {"s21": np.abs(sa.get_s_parameter(1, 0)), "s31": np.abs(sa.get_s_parameter(2, 0)),
+            "p_in": p_in}

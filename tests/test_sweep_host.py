@@ -1,0 +1,80 @@
+"""CPU: the sweep driver and S-parameter host code (no GPU)."""
+
+import numpy as np
+
+from prismo_amd.analysis.sparameters import (SParameterAnalyzer, compute_group_delay,
+                                             export_touchstone)
+from prismo_amd.optimization.sweep import BatchedRunner, ParameterSweep, SweepParameter
+
+
+def test_combinations_are_the_meshgrid_in_order():
+    sw = ParameterSweep([SweepParameter("a", [1, 2]), SweepParameter("b", [10, 20, 30])],
+                        lambda p: {"m": p["a"] * p["b"]})
+    assert [(c["a"], c["b"]) for c in sw.parameter_combinations] == [
+        (1, 10), (1, 20), (1, 30), (2, 10), (2, 20), (2, 30)]
+    res = sw.run()
+    assert [r["m"] for r in res] == [10, 20, 30, 20, 40, 60]
+    assert sw.get_result_array("m").shape == (2, 3)
+    best, r = sw.find_optimal("m")
+    assert best == {"a": 2, "b": 30} and r["m"] == 60
+
+
+def test_errors_are_captured_per_design():
+    def f(p):
+        if p["a"] == 2:
+            raise RuntimeError("boom")
+        return {"m": 1}
+
+    res = ParameterSweep([SweepParameter("a", [1, 2, 3])], f).run()
+    assert [r["status"] for r in res] == ["success", "error", "success"]
+    assert res[1]["error"] == "boom"
+
+
+def test_batched_build_errors_do_not_stop_the_sweep(monkeypatch):
+    import prismo_amd.core.batch as B
+
+    ran = {}
+
+    class FakeEngine:
+        def __init__(self, sims, **kw):
+            ran["n"] = len(sims)
+
+        def run_steps(self, n, graph=True):
+            ran["steps"] = n
+
+    monkeypatch.setattr(B, "BatchEngine", FakeEngine)
+
+    def build(p):
+        if p["a"] < 0:
+            raise ValueError("bad design")
+        return object()
+
+    sw = ParameterSweep.from_samples(
+        [{"a": 1}, {"a": -1}, {"a": 2}],
+        batched=BatchedRunner(build=build, analyze=lambda s, p: {"x": p["a"]}, steps=7))
+    res = sw.run()
+    assert ran == {"n": 2, "steps": 7}
+    assert [r["status"] for r in res] == ["success", "error", "success"]
+    assert res[2]["x"] == 2 and "bad design" in res[1]["error"]
+
+
+def test_s_parameters_and_touchstone(tmp_path):
+    f = np.linspace(1e14, 2e14, 5)
+    sa = SParameterAnalyzer(2, f)
+    pin = np.full(5, 4.0)
+    sa.add_port_data(0, 0, pin, np.full(5, 0.04))
+    sa.add_port_data(1, 0, np.full(5, 1.0), np.zeros(5))
+    assert np.allclose(sa.get_s_parameter(1, 0), 0.5)
+    assert np.allclose(sa.get_s_parameter(0, 0), 0.1)
+    assert np.allclose(sa.get_insertion_loss_db(1, 0), -20 * np.log10(0.5))
+    path = tmp_path / "dev.s2p"
+    export_touchstone(path, f, sa.s_matrix, comments=["test"])
+    lines = path.read_text().splitlines()
+    assert lines[1] == "! test" and lines[2] == "# Hz S RI R 50.0"
+    row = [float(v) for v in lines[3].split()]
+    # frequency, then S11 re/im, S12, S21, S22 (row-major)
+    assert row[0] == f[0] and row[1] == 0.1 and row[5] == 0.5
+    # group delay of a pure delay tau: exp(-i w tau)
+    tau = 3e-15
+    gd = compute_group_delay(f, np.exp(-1j * 2 * np.pi * f * tau))
+    assert np.allclose(gd, tau)
