@@ -130,6 +130,17 @@ class pm_batch_plan(C.Structure):
 
 
 # Exported symbols and their signatures (the full set declared in the headers).
+MLP_LINEAR, MLP_GELU, MLP_DGELU, MLP_PLAIN = 0, 1, 2, 3
+fp = C.POINTER(C.c_float)
+MLP_SIGNATURES = {
+    "pm_mlp_gemm": ([vp, C.c_int, i32, i32, i32, vp, i64, i64, vp, i64, i64, vp, i64, vp, vp],
+                    C.c_int),
+    "pm_mlp_colsum": ([vp, i32, i32, vp, i64, vp], C.c_int),
+    "pm_mlp_mse": ([vp, i32, i32, vp, vp, vp, vp, vp], C.c_int),
+    "pm_mlp_adam": ([vp, i64, vp, vp, vp, vp, C.c_float, C.c_float, C.c_float, C.c_float,
+                     C.c_float, i32], C.c_int),
+}
+
 BATCH_SIGNATURES = {
     "pm_batch_create": ([C.c_int, vp, C.POINTER(vp)], C.c_int),
     "pm_batch_destroy": ([vp], C.c_int),
@@ -181,7 +192,7 @@ def load(path: os.PathLike | None = None):
             "(the mi355x time step has no CPU fallback)"
         )
     lib = C.CDLL(str(p))
-    for name, (args, res) in {**SIGNATURES, **BATCH_SIGNATURES}.items():
+    for name, (args, res) in {**SIGNATURES, **BATCH_SIGNATURES, **MLP_SIGNATURES}.items():
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = res

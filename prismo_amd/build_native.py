@@ -37,6 +37,7 @@ UNITS = {
     "aux.hip": [],
     "dft_mfma.hip": [],
     "batch2d.hip": [],
+    "mlp.hip": [],
     "fused_f32.hip": [],
     "fused_f64.hip": [],
 }

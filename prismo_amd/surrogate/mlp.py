@@ -1,0 +1,181 @@
+"""Spectral surrogate: an MLP trained and evaluated by the project's own
+BF16 MFMA kernels (include/pmmlp.h), with FP32 master weights.
+
+Layers: ``n_in -> hidden x depth (GELU) -> n_out``.  Forward: one GEMM per
+layer with the bias and GELU fused into its epilogue (the pre-activations
+are kept for the backward pass).  Backward: per layer one GEMM for the
+weight gradient (dY^T A over the batch), one column sum for the bias
+gradient and one GEMM for the input gradient with the GELU derivative fused
+into its epilogue.  Loss: weighted MSE.  Optimiser: Adam (decoupled weight
+decay) on the FP32 masters; the GEMMs round operands to BF16 as they stage
+them.  ``SurrogateEnsemble`` trains several members from different seeds;
+their spread is the predictive uncertainty.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from .. import _native as N
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+class SpectralMLP:
+    def __init__(self, n_in: int = 4, n_out: int = 256, hidden: int = 256, depth: int = 4,
+                 seed: int = 0, device: int = 0):
+        torch = _torch()
+        self.lib = N.load()
+        self.device = torch.device("cuda", device)
+        self.sizes = [n_in] + [hidden] * depth + [n_out]
+        rng = np.random.default_rng(seed)
+        self.W, self.b = [], []
+        for k in range(len(self.sizes) - 1):
+            fi, fo = self.sizes[k], self.sizes[k + 1]
+            std = np.sqrt(2.0 / fi) if k < len(self.sizes) - 2 else np.sqrt(1.0 / fi)
+            self.W.append(torch.from_numpy(rng.normal(0, std, (fo, fi)).astype(np.float32)).to(self.device))
+            self.b.append(torch.zeros(fo, dtype=torch.float32, device=self.device))
+        self._adam = None
+        self.t = 0
+
+    # ---------------------------------------------------------------- helpers
+    def _stream(self):
+        return _torch().cuda.current_stream(self.device).cuda_stream
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise N.NativeError(f"{what} failed ({rc})")
+
+    def _gemm(self, mode, M, Nn, K, A, lam, lak, B, lbk, lbn, C, ldc, bias=None, Z=None):
+        rc = self.lib.pm_mlp_gemm(self._stream(), mode, M, Nn, K, A.data_ptr(), lam, lak,
+                                  B.data_ptr(), lbk, lbn, C.data_ptr(), ldc,
+                                  bias.data_ptr() if bias is not None else None,
+                                  Z.data_ptr() if Z is not None else None)
+        self._check(rc, "pm_mlp_gemm")
+
+    def params(self):
+        return [(w.cpu().numpy(), b.cpu().numpy()) for w, b in zip(self.W, self.b)]
+
+    # ---------------------------------------------------------------- passes
+    def forward(self, X, keep: bool = False):
+        """X: (M, n_in) float32 CUDA tensor.  Returns Y, and with ``keep``
+        the cache (inputs of every layer, hidden pre-activations)."""
+        torch = _torch()
+        X = X.contiguous()
+        M = X.shape[0]
+        acts, zs = [X], []
+        a = X
+        L = len(self.W)
+        for k in range(L):
+            fo, fi = self.W[k].shape
+            out = torch.empty((M, fo), dtype=torch.float32, device=self.device)
+            if k < L - 1:
+                z = torch.empty_like(out)
+                # C = A W^T: B[k][n] = W[n][k]
+                self._gemm(N.MLP_GELU, M, fo, fi, a, fi, 1, self.W[k], 1, fi, out, fo,
+                           self.b[k], z)
+                zs.append(z)
+            else:
+                self._gemm(N.MLP_LINEAR, M, fo, fi, a, fi, 1, self.W[k], 1, fi, out, fo,
+                           self.b[k])
+            acts.append(out)
+            a = out
+        return (a, (acts, zs)) if keep else a
+
+    def backward(self, cache, dY):
+        """Gradients [(dW, db)] of sum(dY * Y)."""
+        torch = _torch()
+        acts, zs = cache
+        g = dY.contiguous()
+        M = g.shape[0]
+        grads = [None] * len(self.W)
+        for k in range(len(self.W) - 1, -1, -1):
+            fo, fi = self.W[k].shape
+            dW = torch.empty((fo, fi), dtype=torch.float32, device=self.device)
+            # dW[o][i] = sum_m g[m][o] a[m][i]: A[m'=o][k'=m] = g[m][o]
+            self._gemm(N.MLP_PLAIN, fo, fi, M, g, 1, fo, acts[k], fi, 1, dW, fi)
+            db = torch.empty(fo, dtype=torch.float32, device=self.device)
+            self._check(self.lib.pm_mlp_colsum(self._stream(), M, fo, g.data_ptr(), fo,
+                                               db.data_ptr()), "pm_mlp_colsum")
+            grads[k] = (dW, db)
+            if k > 0:
+                gp = torch.empty((M, fi), dtype=torch.float32, device=self.device)
+                # g_prev[m][i] = (sum_o g[m][o] W[o][i]) * gelu'(z_prev[m][i])
+                self._gemm(N.MLP_DGELU, M, fi, fo, g, fo, 1, self.W[k], fi, 1, gp, fi,
+                           None, zs[k - 1])
+                g = gp
+        return grads
+
+    def mse(self, Y, T, w=None):
+        torch = _torch()
+        dY = torch.empty_like(Y)
+        loss = torch.zeros(1, dtype=torch.float32, device=self.device)
+        M, Nn = Y.shape
+        self._check(self.lib.pm_mlp_mse(self._stream(), M, Nn, Y.data_ptr(), T.data_ptr(),
+                                        w.data_ptr() if w is not None else None,
+                                        dY.data_ptr(), loss.data_ptr()), "pm_mlp_mse")
+        return loss, dY
+
+    def adam_step(self, grads, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8, wd=0.0):
+        torch = _torch()
+        if self._adam is None:
+            self._adam = [(torch.zeros_like(w), torch.zeros_like(w), torch.zeros_like(b),
+                           torch.zeros_like(b)) for w, b in zip(self.W, self.b)]
+        self.t += 1
+        for (W, b), (gW, gb), (mW, vW, mb, vb) in zip(zip(self.W, self.b), grads, self._adam):
+            for p, g, m, v, decay in ((W, gW, mW, vW, wd), (b, gb, mb, vb, 0.0)):
+                self._check(self.lib.pm_mlp_adam(self._stream(), p.numel(), p.data_ptr(),
+                                                 g.data_ptr(), m.data_ptr(), v.data_ptr(),
+                                                 lr, b1, b2, eps, decay, self.t), "pm_mlp_adam")
+
+    def fit(self, X, T, epochs: int = 200, batch: int = 256, lr: float = 2e-3, w=None,
+            seed: int = 0, lr_final: float = 1e-4, wd: float = 0.0):
+        """Minibatch Adam with a cosine learning-rate decay; returns the
+        per-epoch mean loss (host floats)."""
+        torch = _torch()
+        M = X.shape[0]
+        rng = np.random.default_rng(seed)
+        hist = []
+        steps = epochs * ((M + batch - 1) // batch)
+        s = 0
+        for ep in range(epochs):
+            perm = torch.from_numpy(rng.permutation(M)).to(self.device)
+            tot = torch.zeros(1, dtype=torch.float32, device=self.device)
+            nb = 0
+            for i0 in range(0, M, batch):
+                idx = perm[i0:i0 + batch]
+                xb, tb = X[idx], T[idx]
+                Y, cache = self.forward(xb, keep=True)
+                loss, dY = self.mse(Y, tb, w)
+                grads = self.backward(cache, dY)
+                lr_s = lr_final + 0.5 * (lr - lr_final) * (1 + np.cos(np.pi * s / max(1, steps)))
+                self.adam_step(grads, lr=float(lr_s), wd=wd)
+                tot += loss
+                nb += 1
+                s += 1
+            hist.append(float(tot.item()) / nb)
+        return hist
+
+
+class SurrogateEnsemble:
+    """Members trained from different seeds on the same data; prediction =
+    member mean, uncertainty = member standard deviation."""
+
+    def __init__(self, n_members: int = 4, **kw):
+        self.members = [SpectralMLP(seed=1000 + k, **kw) for k in range(n_members)]
+
+    def fit(self, X, T, **kw):
+        return [m.fit(X, T, seed=k, **kw) for k, m in enumerate(self.members)]
+
+    def predict(self, X, chunk: int = 16384):
+        torch = _torch()
+        outs = []
+        for m in self.members:
+            parts = [m.forward(X[i:i + chunk]) for i in range(0, X.shape[0], chunk)]
+            outs.append(torch.cat(parts))
+        S = torch.stack(outs)
+        return S.mean(0), S.std(0)

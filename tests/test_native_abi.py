@@ -24,7 +24,8 @@ def declared_functions():
 
 
 def test_header_declares_what_binding_binds():
-    assert set(declared_functions()) == set(N.SIGNATURES) | set(N.BATCH_SIGNATURES)
+    assert set(declared_functions()) == (set(N.SIGNATURES) | set(N.BATCH_SIGNATURES)
+                                         | set(N.MLP_SIGNATURES))
 
 
 def test_library_loads_and_exports_all_symbols():
