@@ -28,6 +28,11 @@ FP64 runs of an evenly spaced sample of the same designs through
 Simulation.run; ``max_abs_s21_err`` is the largest |S21| difference
 between the two over that sample and all 64 frequencies.
 
+``--surrogate`` runs only the surrogate-screened coupler search (4096
+FDTD training spectra, a 4-member BF16 MLP ensemble, 65536 screened
+designs, FDTD judge of the top 64 and the uncertain ones) and prints its
+own JSON line.
+
 ``--copy-probe`` runs only a 256 MiB device copy (the counter calibration
 kernel); ``--launch vec,ty,kchunk,order`` fixes the fused launch shape
 instead of autotuning.
@@ -217,6 +222,7 @@ def main():
     ap.add_argument("--launch", type=str, default=None)
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--surrogate", action="store_true")
     ap.add_argument("--sweep-only", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -236,6 +242,13 @@ def main():
     pa.set_backend("mi355x", 0)
     if args.sweep_only:
         print(json.dumps({"sweep": sweep_bench()}), flush=True)
+        return
+    if args.surrogate:
+        from prismo_amd.surrogate import loop
+
+        res = loop.run(n_train=4096, n_screen=65536, top_k=64, epochs=300,
+                       log=lambda m: print(m, file=sys.stderr, flush=True))
+        print(json.dumps({"surrogate": res}), flush=True)
         return
     sim = voxel_3d(dtype=np.float32, coefficient_storage="fp16", n=args.n)
     eng = sim.solver.updater.engine_for(sim.fields)

@@ -160,7 +160,8 @@ def coupler_sim(p, dtype=np.float32, backend=None, coefficient_storage="auto",
 
 def coupler_analyze(sim, params=None) -> dict:
     """|S21| (through) and |S31| (cross) from the port fluxes, normalised to
-    the input port's forward power."""
+    the input port's forward power; t21/t31 add the phase of each port's
+    summed Ez relative to the input port's."""
     from .analysis.sparameters import SParameterAnalyzer, port_power
 
     ports = sim.ports
@@ -171,5 +172,13 @@ def coupler_analyze(sim, params=None) -> dict:
     sa.add_port_data(0, 0, p_in, zero)
     sa.add_port_data(1, 0, np.maximum(port_power(ports["thru"], "x", +1), 0.0), zero)
     sa.add_port_data(2, 0, np.maximum(port_power(ports["cross"], "x", +1), 0.0), zero)
-    return {"s21": np.abs(sa.get_s_parameter(1, 0)), "s31": np.abs(sa.get_s_parameter(2, 0)),
-            "p_in": p_in}
+    s21 = np.abs(sa.get_s_parameter(1, 0))
+    s31 = np.abs(sa.get_s_parameter(2, 0))
+
+    def phase(m):
+        return np.angle(np.array([np.sum(m.get_frequency_data("Ez", f)) for f in freqs]))
+
+    ph_in = phase(ports["in"])
+    return {"s21": s21, "s31": s31, "p_in": p_in,
+            "t21": s21 * np.exp(1j * (phase(ports["thru"]) - ph_in)),
+            "t31": s31 * np.exp(1j * (phase(ports["cross"]) - ph_in))}
