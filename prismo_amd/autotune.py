@@ -83,15 +83,16 @@ def tune(eng, steps: int = 6, warmup: int = 2, candidates: dict | None = None,
         for vec, ty, kc, order in product(cand["vec"], cand["ty"], cand["kchunk"], cand["order"]):
             try:
                 eng.ctx.set_tuning(True, ty, kc, vec, order)
-            except ValueError:
+                if not eng.ctx.fused_active():
+                    continue
+                eng.ctx.run(0, warmup if warmup % 2 == 0 else warmup + 1, 0)
+                eng.ctx.set_timing(True)
+                eng.ctx.run(0, steps if steps % 2 == 0 else steps + 1, 0)
+                ms, n = eng.ctx.timing(3)
+                eng.ctx.set_timing(False)
+            except (ValueError, NotImplementedError):
+                eng.ctx.set_timing(False)
                 continue
-            if not eng.ctx.fused_active():
-                continue
-            eng.ctx.run(0, warmup if warmup % 2 == 0 else warmup + 1, 0)
-            eng.ctx.set_timing(True)
-            eng.ctx.run(0, steps if steps % 2 == 0 else steps + 1, 0)
-            ms, n = eng.ctx.timing(3)
-            eng.ctx.set_timing(False)
             if n and (best is None or ms < best["ms"]):
                 best = {"vec": vec, "ty": ty, "kchunk": kc, "order": order, "ms": ms}
     finally:

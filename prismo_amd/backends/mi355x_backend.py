@@ -122,6 +122,24 @@ class MI355XBackend(Backend):
             return buf[0, : shape[1], : shape[0]].permute(1, 0)
         return buf[: shape[2], : shape[1], : shape[0]].permute(2, 1, 0)
 
+    def field_set(self, grid, dtype=None) -> dict:
+        """All six components in one zeroed allocation, Ex Ey Ez Hx Hy Hz at a
+        constant stride (the layout the fused step kernel addresses), each
+        returned as its reference-shaped view."""
+        torch = _torch()
+        nx, ny, nz = grid.dimensions
+        arena = torch.zeros((6, nz, ny, pitch_for(nx)), dtype=torch_dtype(dtype),
+                            device=self.device)
+        out = {}
+        for n, c in enumerate(("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")):
+            shape = grid.get_field_shape(c)
+            buf = arena[n]
+            if grid.is_2d:
+                out[c] = buf[0, : shape[1], : shape[0]].permute(1, 0)
+            else:
+                out[c] = buf[: shape[2], : shape[1], : shape[0]].permute(2, 1, 0)
+        return out
+
     # -- math ------------------------------------------------------------------
     def sqrt(self, a):
         return _torch().sqrt(a)

@@ -83,6 +83,7 @@ class StepEngine:
         self._bindings: dict = {}
         self.plan = N.pm_plan()
         self._fill_grid()
+        self._make_arena(cpml)
         self._fill_coefficients(coeffs, coefficient_storage)
         self._fill_cpml(cpml)
         self._fill_ade(list(ade))
@@ -102,12 +103,43 @@ class StepEngine:
         self._keep.append(t)
         return t
 
-    def _padded(self, host: np.ndarray, comp: str, dtype):
+    def _make_arena(self, cpml):
+        """One zeroed allocation for the coefficient, CPML memory and profile
+        arrays, so they sit within a few hundred MB of each other: the fused
+        kernel addresses them with 32-bit offsets from one base."""
+        nx, ny, nz = self.grid.dimensions
+        esz = 8 if self.precision == N.FP64 else 4
+        field = nz * ny * self.sy * esz
+        total = 3 * field + 6 * (nx + ny + nz) * esz + 4096
+        if cpml is not None:
+            cells = nx * ny * nz
+            for a, n in enumerate((nx, ny, nz)):
+                S = cpml.thickness[2 * a] + cpml.thickness[2 * a + 1]
+                total += 6 * (S * cells // n) * esz + 6 * 256
+        self._arena = _torch().zeros((total,), dtype=_torch().uint8, device=self.device)
+        self._arena_off = 0
+
+    def _carve(self, shape, dtype):
+        """Zeroed array from the arena (256-byte aligned), else a plain one."""
+        torch = _torch()
+        n = int(np.prod(shape))
+        nbytes = n * torch.empty((), dtype=dtype).element_size()
+        start = (self._arena_off + 255) // 256 * 256
+        if start + nbytes > self._arena.numel():
+            return self._alloc(shape, dtype)
+        t = self._arena[start:start + nbytes].view(dtype).view(shape)
+        self._arena_off = start + nbytes
+        self._keep.append(t)
+        return t
+
+    def _padded(self, host: np.ndarray, comp: str, dtype, buf=None):
         """Device array in the shared padded layout holding ``host`` (shaped
-        like the component) at its reference indices, zero elsewhere."""
+        like the component) at its reference indices, zero elsewhere (into
+        ``buf`` when given)."""
         torch = _torch()
         nx, ny, nz = self.grid.dimensions
-        buf = self._alloc((nz, ny, self.sy), dtype)
+        if buf is None:
+            buf = self._alloc((nz, ny, self.sy), dtype)
         if self.is_2d:
             view = buf[0, : host.shape[1], : host.shape[0]].permute(1, 0)
         else:
@@ -179,17 +211,22 @@ class StepEngine:
                 raise ValueError("uniform Db expected for lossless non-magnetic media")
             p.db_const = d0
         self.cb_host = {}
+        nx, ny, nz = self.grid.dimensions
+        cb_dt = torch.float16 if kind == N.COEFF_CB_F16 else self.tdtype
+        # the three Cb arrays back to back (the fused kernel's layout)
+        cb_all = None if self.is_2d else self._carve((3, nz, ny, self.sy), cb_dt)
         for a, c in enumerate(("Ex", "Ey", "Ez")):
             if c not in self.need_e:
                 continue
             cb = component_average(sl(Cb), c, self.is_2d)
+            dst = None if cb_all is None else cb_all[a]
             if kind == N.COEFF_CB_F16:
                 q = (cb / self.cb_scale).astype(np.float16)
                 self.cb_host[c] = q.astype(np.float64) * self.cb_scale
-                p.cb[a] = self._padded(q, c, torch.float16).data_ptr()
+                p.cb[a] = self._padded(q, c, torch.float16, dst).data_ptr()
             else:
                 self.cb_host[c] = cb
-                p.cb[a] = self._padded(cb, c, self.tdtype).data_ptr()
+                p.cb[a] = self._padded(cb, c, self.tdtype, dst).data_ptr()
             if kind == N.COEFF_FULL:
                 p.ca[a] = self._padded(component_average(sl(Ca), c, self.is_2d), c,
                                        self.tdtype).data_ptr()
@@ -217,19 +254,28 @@ class StepEngine:
                 continue
             pr = prof[a]
             t = _torch()
-            for key, attr in (("be", "pml_be"), ("ce", "pml_ce"), ("ike", "pml_ike"),
-                              ("bh", "pml_bh"), ("ch", "pml_ch"), ("ikh", "pml_ikh")):
-                arr = self._alloc((n[a],), self.tdtype)
-                arr.copy_(t.from_numpy(pr[key]).to(self.device, self.tdtype))
-                getattr(p, attr)[a] = arr.data_ptr()
+            # the six profiles of an axis back to back, then the two CPML
+            # memory arrays of each side back to back (the fused kernel's layout)
+            keys = (("be", "pml_be"), ("ce", "pml_ce"), ("ike", "pml_ike"),
+                    ("bh", "pml_bh"), ("ch", "pml_ch"), ("ikh", "pml_ikh"))
+            pbuf = self._carve((6, n[a]), self.tdtype)
+            pbuf.copy_(t.from_numpy(np.stack([pr[k] for k, _ in keys])).to(self.device, self.tdtype))
+            for r, (_, attr) in enumerate(keys):
+                getattr(p, attr)[a] = pbuf[r].data_ptr()
             S = tl + th
             shape = {0: (nz, ny, S), 1: (nz, S, nx), 2: (S, ny, nx)}[a]
-            for q in range(2):
-                comp_idx = (a + 2) % 3 if q == 1 else (a + 1) % 3
-                if ("Ex", "Ey", "Ez")[comp_idx] in self.need_e:
-                    p.psi_e[a][q] = self._alloc(shape, self.tdtype).data_ptr()
-                if ("Hx", "Hy", "Hz")[comp_idx] in self.need_h:
-                    p.psi_h[a][q] = self._alloc(shape, self.tdtype).data_ptr()
+            for side, need, comps in ((p.psi_e, self.need_e, ("Ex", "Ey", "Ez")),
+                                      (p.psi_h, self.need_h, ("Hx", "Hy", "Hz"))):
+                want = [comps[(a + 2) % 3 if q == 1 else (a + 1) % 3] in need for q in range(2)]
+                if all(want):
+                    pair = self._carve((2,) + shape, self.tdtype)
+                    for q in range(2):
+                        self._keep.append(pair[q])
+                        side[a][q] = pair[q].data_ptr()
+                else:
+                    for q in range(2):
+                        if want[q]:
+                            side[a][q] = self._alloc(shape, self.tdtype).data_ptr()
 
     def _fill_fused(self, fused: bool):
         """Second buffer set for the fused ping-pong step (3-D yee, one x-row
@@ -241,14 +287,24 @@ class StepEngine:
         if self.sy not in (128, 256, 512):
             return
         nx, ny, nz = self.grid.dimensions
+        # second field set: Ex Ey Ez Hx Hy Hz back to back at the stride of
+        # the first set (ElectromagneticFields allocates it the same way)
+        setb = self._alloc((6, nz, ny, self.sy), self.tdtype)
         for a in range(3):
-            p.E2[a] = self._alloc((nz, ny, self.sy), self.tdtype).data_ptr()
-            p.H2[a] = self._alloc((nz, ny, self.sy), self.tdtype).data_ptr()
+            p.E2[a] = setb[a].data_ptr()
+            p.H2[a] = setb[3 + a].data_ptr()
         for a in range(3):
-            for q in range(2):
-                if p.psi_h[a][q]:
-                    src = next(t for t in self._keep if t.data_ptr() == p.psi_h[a][q])
-                    p.psi_h2[a][q] = self._alloc(tuple(src.shape), self.tdtype).data_ptr()
+            S = p.pml[2 * a] + p.pml[2 * a + 1]
+            shape = {0: (nz, ny, S), 1: (nz, S, nx), 2: (S, ny, nx)}[a]
+            if p.psi_h[a][0] and p.psi_h[a][1]:
+                pair = self._carve((2,) + shape, self.tdtype)
+                for q in range(2):
+                    self._keep.append(pair[q])
+                    p.psi_h2[a][q] = pair[q].data_ptr()
+            else:
+                for q in range(2):
+                    if p.psi_h[a][q]:
+                        p.psi_h2[a][q] = self._alloc(shape, self.tdtype).data_ptr()
         self.fused = True
 
     def _fill_ade(self, regions):

@@ -33,6 +33,9 @@ class ElectromagneticFields:
 
     def _initialize_fields(self) -> None:
         b = self.backend
+        if hasattr(b, "field_set"):
+            self._fields.update(b.field_set(self.grid, self.dtype))
+            return
         for c in COMPONENTS:
             if hasattr(b, "field_zeros"):
                 self._fields[c] = b.field_zeros(self.grid, c, self.dtype)

@@ -277,12 +277,47 @@ int do_e(pm_ctx* c) {
 }
 
 // Cells per lane and waves per row of the fused kernel for this plan.
-void fused_shape(const pm_ctx* c, int* vec, int* w) {
+// Launch shape of the fused kernel: the tuned one, else (rows per workgroup
+// and then cells per lane relaxed) the first that fits the workgroup and LDS
+// limits.  False if none does.
+bool fused_shape(const pm_ctx* c, int* vec, int* w, int* ty) {
   const int rw = (int)c->plan.sy;  // padded row, 128 / 256 / 512
-  int v = c->tune_vec;
-  if (v <= 0) v = c->plan.precision == PM_FP32 ? (rw >= 128 ? 2 : 1) : (rw <= 256 ? 1 : 2);
-  *vec = v;
-  *w = rw / (64 * v);
+  const int prec = c->plan.precision;
+  int prefer = c->tune_vec;
+  if (prefer <= 0) prefer = prec == PM_FP32 ? (rw >= 128 ? 2 : 1) : (rw <= 256 ? 1 : 2);
+  const int vecs[4] = {prefer, 1, 2, 4};
+  const int nv = c->tune_vec > 0 ? 1 : 4;
+  for (int iv = 0; iv < nv; ++iv) {
+    const int v = vecs[iv];
+    if (rw % (64 * v)) continue;
+    for (int t : {c->tune_ty, 2}) {
+      if (fused_supported(prec, v, rw / (64 * v), t)) {
+        *vec = v;
+        *w = rw / (64 * v);
+        *ty = t;
+        return true;
+      }
+    }
+  }
+  *vec = prefer;
+  *w = rw / (64 * prefer);
+  *ty = c->tune_ty;
+  return false;
+}
+
+FusedBufs fused_bufs(const pm_plan& p, bool a_in) {
+  FusedBufs fb{};
+  for (int a = 0; a < 3; ++a) {
+    fb.Ei[a] = a_in ? p.E[a] : p.E2[a];
+    fb.Hi[a] = a_in ? p.H[a] : p.H2[a];
+    fb.Eo[a] = a_in ? p.E2[a] : p.E[a];
+    fb.Ho[a] = a_in ? p.H2[a] : p.H[a];
+    for (int q = 0; q < 2; ++q) {
+      fb.psh_in[a][q] = a_in ? p.psi_h[a][q] : p.psi_h2[a][q];
+      fb.psh_out[a][q] = a_in ? p.psi_h2[a][q] : p.psi_h[a][q];
+    }
+  }
+  return fb;
 }
 
 // One fused step from buffer set `parity` into the other one.
@@ -301,16 +336,15 @@ int do_fused(pm_ctx* c) {
       fb.psh_out[a][q] = a_in ? p.psi_h2[a][q] : p.psi_h[a][q];
     }
   }
-  int vec, w;
-  fused_shape(c, &vec, &w);
+  int vec, w, ty;
+  fused_shape(c, &vec, &w, &ty);
   const int r = p.precision == PM_FP32
-                    ? launch_yee_fused_f32(c->g, p.coeff_kind, fb, vec, w, c->tune_ty,
+                    ? launch_yee_fused_f32(c->g, p.coeff_kind, fb, vec, w, ty,
                                            c->tune_kchunk, c->tune_order, c->stream)
-                    : launch_yee_fused_f64(c->g, p.coeff_kind, fb, vec, w, c->tune_ty,
+                    : launch_yee_fused_f64(c->g, p.coeff_kind, fb, vec, w, ty,
                                            c->tune_kchunk, c->tune_order, c->stream);
   if (r != 0)
-    return fail(c, PM_ENOTSUP, "fused step: no kernel for vec %d, w %d, ty %d", vec, w,
-                c->tune_ty);
+    return fail(c, PM_ENOTSUP, "fused step: no kernel for vec %d, w %d, ty %d", vec, w, ty);
   c->parity ^= 1;
   return launch_status(c, "fused step launch");
 }
@@ -494,7 +528,9 @@ int pm_configure(pm_ctx* c, const pm_plan* plan) {
     c->aux2.F[a] = p.E2[a];
     c->aux2.F[3 + a] = p.H2[a];
   }
-  c->fused_ok = fused_eligible(p);
+  c->fused_ok = fused_eligible(p) &&
+                fused_addressable(c->g, fused_bufs(p, true), p.precision == PM_FP64 ? 8 : 4) &&
+                fused_addressable(c->g, fused_bufs(p, false), p.precision == PM_FP64 ? 8 : 4);
   c->parity = 0;
   c->pend_n = 0;
   c->configured = true;
@@ -529,10 +565,8 @@ int pm_run(pm_ctx* c, int64_t step0, int32_t nsteps, uint32_t flags) {
                   (long long)p.table_step0, (long long)(p.table_step0 + p.table_len),
                   (long long)step0, (long long)(step0 + nsteps));
   }
-  int fvec = 0, fw = 0;
-  if (c->fused_ok) fused_shape(c, &fvec, &fw);
-  const bool fused =
-      c->fused_ok && c->fuse && fused_supported(p.precision, fvec, fw, c->tune_ty);
+  int fvec = 0, fw = 0, fty = 0;
+  const bool fused = c->fused_ok && c->fuse && fused_shape(c, &fvec, &fw, &fty);
   for (int32_t s = 0; s < nsteps; ++s) {
     const int64_t n = step0 + s;
     Span whole(c, 2);
@@ -602,10 +636,9 @@ int pm_set_tuning(pm_ctx* c, int fuse, int ty, int kchunk, int vec, int order) {
 
 int pm_fused_active(pm_ctx* c) {
   if (!c || !c->configured) return 0;
-  int vec = 0, w = 0;
+  int vec = 0, w = 0, ty = 0;
   if (!c->fused_ok) return 0;
-  fused_shape(c, &vec, &w);
-  return (c->fuse && fused_supported(c->plan.precision, vec, w, c->tune_ty)) ? 1 : 0;
+  return (c->fuse && fused_shape(c, &vec, &w, &ty)) ? 1 : 0;
 }
 
 int pm_sync(pm_ctx* c) {

@@ -5,7 +5,8 @@ namespace pm {
 
 int launch_yee_fused_f64(const KGrid& g, int ck, const FusedBufs& fb, int vec, int w, int ty,
                          int kchunk, int order, hipStream_t s) {
-  const Fz<double> z = make_fz<double>(fb);
+  KFz z;
+  if (!make_kfz(g, fb, (int)sizeof(double), &z)) return -1;
   if (ck == PM_COEFF_FULL)
     return launch_fused_ck<double, PM_COEFF_FULL>(g, z, vec, w, ty, kchunk, order, s);
   return launch_fused_ck<double, PM_COEFF_CB>(g, z, vec, w, ty, kchunk, order, s);

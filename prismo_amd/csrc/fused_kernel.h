@@ -21,19 +21,41 @@
 //  - workgroups map to XCDs in contiguous ranges of (y-tile, z-chunk) in the
 //    order picked by the tuner, so shared halo rows / planes are L2 hits.
 #pragma once
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+
 #include "pm_internal.h"
 
 namespace pm {
 
-template <typename T>
-struct Fz {
-  const T* Ei[3];
-  const T* Hi[3];
-  T* Eo[3];
-  T* Ho[3];
-  const T* psh_in[3][2];
-  T* psh_out[3][2];
+// Every array the fused kernel touches, as 32-bit byte offsets from one of
+// three base addresses: the field set read this step (bi), the set written
+// (bo), and everything else (bc: coefficients, CPML memory and profiles).
+// The launcher checks each group spans less than 4 GiB.  One SGPR per array
+// instead of a pointer pair keeps the loop's uniform state in registers.
+struct KFz {
+  const char* bi;
+  const char* bo;
+  const char* bc;
+  uint32_t cs;      // bytes from one field component to the next (Ex Ey Ez Hx Hy Hz)
+  uint32_t cb, ccs; // Cb of Ex; bytes between the three Cb arrays
+  uint32_t ca[3], da[3], db[3];
+  uint32_t pe[3], phi[3], pho[3];  // CPML memory per axis: q = 0 then q = 1
+  uint32_t ps[3];                  // bytes of one CPML memory array of axis a
+  uint32_t prof[3];                // per axis: be ce ike bh ch ikh, n_a each
+  uint32_t pn[3];                  // bytes of one profile array of axis a
 };
+
+template <typename T>
+__device__ __forceinline__ const T* zp(const char* base, uint32_t off, uint32_t elem) {
+  return (const T*)(base + (off + elem * (uint32_t)sizeof(T)));
+}
+
+template <typename T>
+__device__ __forceinline__ T* zpw(const char* base, uint32_t off, uint32_t elem) {
+  return (T*)(base + (off + elem * (uint32_t)sizeof(T)));
+}
 
 __device__ __forceinline__ int fslab(int p, int n, int tlo, int thi) {
   if (p < tlo) return p;
@@ -41,59 +63,25 @@ __device__ __forceinline__ int fslab(int p, int n, int tlo, int thi) {
   return -1;
 }
 
-__device__ __forceinline__ int64_t fslab_off(const KGrid& g, int a, int s, int i, int j, int k) {
+__device__ __forceinline__ uint32_t fslab_off(const KGrid& g, int a, int s, int i, int j, int k) {
   if (a == 0) {
-    const int S = g.pml[0] + g.pml[1];
-    return (int64_t)s + (int64_t)S * ((int64_t)j + (int64_t)g.ny * k);
+    const uint32_t S = (uint32_t)(g.pml[0] + g.pml[1]);
+    return (uint32_t)s + S * ((uint32_t)j + (uint32_t)g.ny * (uint32_t)k);
   } else if (a == 1) {
-    const int S = g.pml[2] + g.pml[3];
-    return (int64_t)i + (int64_t)g.nx * ((int64_t)s + (int64_t)S * k);
+    const uint32_t S = (uint32_t)(g.pml[2] + g.pml[3]);
+    return (uint32_t)i + (uint32_t)g.nx * ((uint32_t)s + S * (uint32_t)k);
   }
-  return (int64_t)i + (int64_t)g.nx * ((int64_t)j + (int64_t)g.ny * s);
-}
-
-// H-side CPML term, ping-pong memory; `store` false for halo cells.
-template <typename T>
-__device__ __forceinline__ T fcpml_h(const KGrid& g, const Fz<T>& z, T d, int a, int q, int p,
-                                     int s, int i, int j, int k, bool store) {
-  const T* b = (const T*)g.pml_bh[a];
-  const T* c = (const T*)g.pml_ch[a];
-  const T* ik = (const T*)g.pml_ikh[a];
-  const int64_t o = fslab_off(g, a, s, i, j, k);
-  const T ps = b[p] * z.psh_in[a][q][o] + c[p] * d;
-  if (store) z.psh_out[a][q][o] = ps;
-  return d * ik[p] + ps;
-}
-
-// As fcpml_h with the memory value already loaded (offset o precomputed).
-template <typename T>
-__device__ __forceinline__ T fcpml_hq(const KGrid& g, const Fz<T>& z, T d, T psi_old, int a,
-                                      int q, int p, int64_t o, bool store) {
-  const T ps = ((const T*)g.pml_bh[a])[p] * psi_old + ((const T*)g.pml_ch[a])[p] * d;
-  if (store) z.psh_out[a][q][o] = ps;
-  return d * ((const T*)g.pml_ikh[a])[p] + ps;
-}
-
-// E-side term with the memory value already loaded; stores in place.
-template <typename T>
-__device__ __forceinline__ T fcpml_eq(const KGrid& g, T d, T psi_old, int a, int q, int p,
-                                      int64_t o) {
-  const T ps = ((const T*)g.pml_be[a])[p] * psi_old + ((const T*)g.pml_ce[a])[p] * d;
-  ((T*)g.psi_e[a][q])[o] = ps;
-  return d * ((const T*)g.pml_ike[a])[p] + ps;
+  return (uint32_t)i + (uint32_t)g.nx * ((uint32_t)j + (uint32_t)g.ny * (uint32_t)s);
 }
 
 // E-side CPML term, in place (E is computed on own cells only).
 template <typename T>
-__device__ __forceinline__ T fcpml_e(const KGrid& g, T d, int a, int q, int p, int s, int i,
-                                     int j, int k) {
-  const T* b = (const T*)g.pml_be[a];
-  const T* c = (const T*)g.pml_ce[a];
-  const T* ik = (const T*)g.pml_ike[a];
-  T* psi = (T*)g.psi_e[a][q] + fslab_off(g, a, s, i, j, k);
-  const T ps = b[p] * (*psi) + c[p] * d;
+__device__ __forceinline__ T fcpml_e(const KGrid& g, const KFz& z, T d, int a, int q, int p,
+                                     int s, int i, int j, int k) {
+  T* psi = zpw<T>(z.bc, z.pe[a] + (uint32_t)(q) * z.ps[a], fslab_off(g, a, s, i, j, k));
+  const T ps = *zp<T>(z.bc, z.prof[a] + 0u * z.pn[a], p) * (*psi) + *zp<T>(z.bc, z.prof[a] + 1u * z.pn[a], p) * d;
   *psi = ps;
-  return d * ik[p] + ps;
+  return d * *zp<T>(z.bc, z.prof[a] + 2u * z.pn[a], p) + ps;
 }
 
 // Dispersive E update from the device box table (see yee.hip for the math).
@@ -166,9 +154,10 @@ __device__ __forceinline__ void fzero(T (&d)[VEC]) {
 }
 
 template <typename T, int CK, int VEC>
-__device__ __forceinline__ void fcb(const KGrid& g, int c, uint32_t off, T (&cb)[VEC]) {
+__device__ __forceinline__ void fcb(const KGrid& g, const KFz& z, int c, uint32_t off,
+                                    T (&cb)[VEC]) {
   if constexpr (CK == PM_COEFF_CB_F16) {
-    const _Float16* h = (const _Float16*)g.cb[c] + off;
+    const _Float16* h = (const _Float16*)(z.bc + (z.cb + (uint32_t)c * z.ccs + off * 2u));
     const T s = (T)g.cb_scale;
     if constexpr (VEC == 1) {
       cb[0] = s * (T)h[0];
@@ -179,15 +168,24 @@ __device__ __forceinline__ void fcb(const KGrid& g, int c, uint32_t off, T (&cb)
       for (int e = 0; e < VEC; ++e) cb[e] = s * (T)v[e];
     }
   } else {
-    fld<T, VEC>(cb, (const T*)g.cb[c] + off);
+    fld<T, VEC>(cb, zp<T>(z.bc, z.cb + (uint32_t)c * z.ccs, off));
   }
 }
 
-template <typename T, int CK, int VEC, int W, int TY>
-__global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, const Fz<T> z,
+// Tiles whose rows and planes all lie outside the y and z CPML slabs form a
+// box [ty0, ty1) x [kc0, kc1) of (y-tile, z-chunk); they run in a kernel
+// instance built without the y/z CPML code (YZ = false), the rest in one
+// with it.  Keeping the rarely used slab state out of the common instance
+// keeps its uniform values in SGPRs.
+struct TileBox {
+  int ty0, ty1, kc0, kc1;
+};
+
+template <typename T, int CK, int VEC, int W, int TY, bool YZ>
+__global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, const KFz z,
                                                                  int kchunk, int ntile_y,
                                                                  int nchunk, int nblocks,
-                                                                 int order) {
+                                                                 int order, TileBox bx) {
   constexpr int RW = 64 * VEC * W;
   __shared__ T le[3][TY + 1][RW];     // E(n) of plane k
   __shared__ T lh[2][3][TY + 1][RW];  // H(n+1/2) of planes with parity 0 / 1
@@ -199,12 +197,32 @@ __global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, 
   const int L = order == 2 ? b : (b % 8) * per + b / 8;
   if (L >= nblocks) return;
   int ty, kc;
-  if (order != 1) {
-    ty = L % ntile_y;
-    kc = L / ntile_y;
+  if constexpr (!YZ) {
+    const int nty = bx.ty1 - bx.ty0, nkc = bx.kc1 - bx.kc0;
+    if (order != 1) {
+      ty = bx.ty0 + L % nty;
+      kc = bx.kc0 + L / nty;
+    } else {
+      kc = bx.kc0 + L % nkc;
+      ty = bx.ty0 + L / nkc;
+    }
   } else {
-    kc = L % nchunk;
-    ty = L / nchunk;
+    // every tile outside the box: chunks below it, chunks above it, then
+    // the y-tiles beside it in the chunks it spans
+    const int n_lo = bx.kc0 * ntile_y;
+    const int n_hi = (nchunk - bx.kc1) * ntile_y;
+    if (L < n_lo) {
+      kc = L / ntile_y;
+      ty = L % ntile_y;
+    } else if (L < n_lo + n_hi) {
+      kc = bx.kc1 + (L - n_lo) / ntile_y;
+      ty = (L - n_lo) % ntile_y;
+    } else {
+      const int nb = ntile_y - (bx.ty1 - bx.ty0);
+      const int r = (L - n_lo - n_hi) % nb;
+      kc = bx.kc0 + (L - n_lo - n_hi) / nb;
+      ty = r < bx.ty0 ? r : bx.ty1 + (r - bx.ty0);
+    }
   }
 
   const int tid = threadIdx.x;
@@ -227,7 +245,7 @@ __global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, 
   const bool pml = g.has_pml;
 
   // ---- CPML profiles: x per lane (registers), y per row (scalars) -------
-  const int sys = (pml && row_ok) ? fslab(j, ny, g.pml[2], g.pml[3]) : -1;
+  const int sys = (YZ && pml && row_ok) ? fslab(j, ny, g.pml[2], g.pml[3]) : -1;
   int sxs[VEC];
   bool lane_pml_x = false;
   T xbh[VEC], xch[VEC], xikh[VEC], xbe[VEC], xce[VEC], xike[VEC];
@@ -240,52 +258,46 @@ __global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, 
     xch[e] = xce[e] = (T)0;
     xikh[e] = xike[e] = (T)1;
     if (sxs[e] >= 0) {
-      xbh[e] = ((const T*)g.pml_bh[0])[i];
-      xch[e] = ((const T*)g.pml_ch[0])[i];
-      xikh[e] = ((const T*)g.pml_ikh[0])[i];
-      xbe[e] = ((const T*)g.pml_be[0])[i];
-      xce[e] = ((const T*)g.pml_ce[0])[i];
-      xike[e] = ((const T*)g.pml_ike[0])[i];
+      xbh[e] = *zp<T>(z.bc, z.prof[0] + 3u * z.pn[0], i);
+      xch[e] = *zp<T>(z.bc, z.prof[0] + 4u * z.pn[0], i);
+      xikh[e] = *zp<T>(z.bc, z.prof[0] + 5u * z.pn[0], i);
+      xbe[e] = *zp<T>(z.bc, z.prof[0] + 0u * z.pn[0], i);
+      xce[e] = *zp<T>(z.bc, z.prof[0] + 1u * z.pn[0], i);
+      xike[e] = *zp<T>(z.bc, z.prof[0] + 2u * z.pn[0], i);
     }
   }
   T ybh = 1, ych = 0, yikh = 1, ybe = 1, yce = 0, yike = 1;
   if (sys >= 0) {
-    ybh = ((const T*)g.pml_bh[1])[j];
-    ych = ((const T*)g.pml_ch[1])[j];
-    yikh = ((const T*)g.pml_ikh[1])[j];
-    ybe = ((const T*)g.pml_be[1])[j];
-    yce = ((const T*)g.pml_ce[1])[j];
-    yike = ((const T*)g.pml_ike[1])[j];
+    ybh = *zp<T>(z.bc, z.prof[1] + 3u * z.pn[1], j);
+    ych = *zp<T>(z.bc, z.prof[1] + 4u * z.pn[1], j);
+    yikh = *zp<T>(z.bc, z.prof[1] + 5u * z.pn[1], j);
+    ybe = *zp<T>(z.bc, z.prof[1] + 0u * z.pn[1], j);
+    yce = *zp<T>(z.bc, z.prof[1] + 1u * z.pn[1], j);
+    yike = *zp<T>(z.bc, z.prof[1] + 2u * z.pn[1], j);
   }
-  const T* __restrict__ zbh = (const T*)g.pml_bh[2];
-  const T* __restrict__ zch = (const T*)g.pml_ch[2];
-  const T* __restrict__ zikh = (const T*)g.pml_ikh[2];
-  const T* __restrict__ zbe = (const T*)g.pml_be[2];
-  const T* __restrict__ zce = (const T*)g.pml_ce[2];
-  const T* __restrict__ zike = (const T*)g.pml_ike[2];
 
   // H-side CPML memory of one plane: [x0 x1 y0 y1 z0 z1] per cell
   auto load_qh = [&](int k, T (&q)[6][VEC]) {
-    const int sk = (pml && row_ok) ? fslab(k, nz, g.pml[4], g.pml[5]) : -1;
+    const int sk = (YZ && pml && row_ok) ? fslab(k, nz, g.pml[4], g.pml[5]) : -1;
 #pragma unroll
     for (int e = 0; e < VEC; ++e) {
       const int i = x0 + e;
 #pragma unroll
       for (int t = 0; t < 6; ++t) q[t][e] = (T)0;
       if (sxs[e] >= 0) {
-        const int64_t o = fslab_off(g, 0, sxs[e], i, j, k);
-        q[0][e] = z.psh_in[0][0][o];
-        q[1][e] = z.psh_in[0][1][o];
+        const uint32_t o = fslab_off(g, 0, sxs[e], i, j, k);
+        q[0][e] = *zp<T>(z.bc, z.phi[0] + (uint32_t)(0) * z.ps[0], o);
+        q[1][e] = *zp<T>(z.bc, z.phi[0] + (uint32_t)(1) * z.ps[0], o);
       }
       if (sys >= 0 && i < nx) {
-        const int64_t o = fslab_off(g, 1, sys, i, j, k);
-        q[2][e] = z.psh_in[1][0][o];
-        q[3][e] = z.psh_in[1][1][o];
+        const uint32_t o = fslab_off(g, 1, sys, i, j, k);
+        q[2][e] = *zp<T>(z.bc, z.phi[1] + (uint32_t)(0) * z.ps[1], o);
+        q[3][e] = *zp<T>(z.bc, z.phi[1] + (uint32_t)(1) * z.ps[1], o);
       }
       if (sk >= 0 && i < nx) {
-        const int64_t o = fslab_off(g, 2, sk, i, j, k);
-        q[4][e] = z.psh_in[2][0][o];
-        q[5][e] = z.psh_in[2][1][o];
+        const uint32_t o = fslab_off(g, 2, sk, i, j, k);
+        q[4][e] = *zp<T>(z.bc, z.phi[2] + (uint32_t)(0) * z.ps[2], o);
+        q[5][e] = *zp<T>(z.bc, z.phi[2] + (uint32_t)(1) * z.ps[2], o);
       }
     }
   };
@@ -305,20 +317,20 @@ __global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, 
   fzero(ezu);
   if (row_ok && kb > 0) {
     const uint32_t o = rowoff + (uint32_t)(kb - 1) * sz;
-    fld(exm, z.Ei[0] + o);
-    fld(eym, z.Ei[1] + o);
+    fld(exm, zp<T>(z.bi, (uint32_t)(0) * z.cs, o));
+    fld(eym, zp<T>(z.bi, (uint32_t)(1) * z.cs, o));
   }
   if (row_ok) {
     const uint32_t o = rowoff + (uint32_t)kb * sz;
-    fld(ex, z.Ei[0] + o);
-    fld(ey, z.Ei[1] + o);
-    fld(ez, z.Ei[2] + o);
-    fld(hxi, z.Hi[0] + o);
-    fld(hyi, z.Hi[1] + o);
-    fld(hzi, z.Hi[2] + o);
+    fld(ex, zp<T>(z.bi, (uint32_t)(0) * z.cs, o));
+    fld(ey, zp<T>(z.bi, (uint32_t)(1) * z.cs, o));
+    fld(ez, zp<T>(z.bi, (uint32_t)(2) * z.cs, o));
+    fld(hxi, zp<T>(z.bi, (uint32_t)(3 + 0) * z.cs, o));
+    fld(hyi, zp<T>(z.bi, (uint32_t)(3 + 1) * z.cs, o));
+    fld(hzi, zp<T>(z.bi, (uint32_t)(3 + 2) * z.cs, o));
     if (first_row) {
-      fld(exu, z.Ei[0] + o - sy);
-      fld(ezu, z.Ei[2] + o - sy);
+      fld(exu, zp<T>(z.bi, (uint32_t)(0) * z.cs, o - sy));
+      fld(ezu, zp<T>(z.bi, (uint32_t)(2) * z.cs, o - sy));
     }
   } else {
     fzero(ex);
@@ -339,11 +351,11 @@ __global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, 
     //     later wait on these does not wait on the prefetch
     T cbx[VEC], cby[VEC], cbz[VEC];
     T qe[6][VEC];
-    const int szz = (pml && do_e) ? fslab(kk, nz, g.pml[4], g.pml[5]) : -1;
+    const int szz = (YZ && pml && do_e) ? fslab(kk, nz, g.pml[4], g.pml[5]) : -1;
     if (do_e) {
-      fcb<T, CK, VEC>(g, 0, off - sz, cbx);
-      fcb<T, CK, VEC>(g, 1, off - sz, cby);
-      fcb<T, CK, VEC>(g, 2, off - sz, cbz);
+      fcb<T, CK, VEC>(g, z, 0, off - sz, cbx);
+      fcb<T, CK, VEC>(g, z, 1, off - sz, cby);
+      fcb<T, CK, VEC>(g, z, 2, off - sz, cbz);
     }
     if (pml) {
 #pragma unroll
@@ -353,19 +365,19 @@ __global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, 
         for (int t = 0; t < 6; ++t) qe[t][e] = (T)0;
         if (!do_e || i >= nx) continue;
         if (sxs[e] >= 0) {
-          const int64_t o = fslab_off(g, 0, sxs[e], i, j, kk);
-          qe[0][e] = ((const T*)g.psi_e[0][0])[o];
-          qe[1][e] = ((const T*)g.psi_e[0][1])[o];
+          const uint32_t o = fslab_off(g, 0, sxs[e], i, j, kk);
+          qe[0][e] = *zp<T>(z.bc, z.pe[0] + (uint32_t)(0) * z.ps[0], o);
+          qe[1][e] = *zp<T>(z.bc, z.pe[0] + (uint32_t)(1) * z.ps[0], o);
         }
         if (sys >= 0) {
-          const int64_t o = fslab_off(g, 1, sys, i, j, kk);
-          qe[2][e] = ((const T*)g.psi_e[1][0])[o];
-          qe[3][e] = ((const T*)g.psi_e[1][1])[o];
+          const uint32_t o = fslab_off(g, 1, sys, i, j, kk);
+          qe[2][e] = *zp<T>(z.bc, z.pe[1] + (uint32_t)(0) * z.ps[1], o);
+          qe[3][e] = *zp<T>(z.bc, z.pe[1] + (uint32_t)(1) * z.ps[1], o);
         }
         if (szz >= 0) {
-          const int64_t o = fslab_off(g, 2, szz, i, j, kk);
-          qe[4][e] = ((const T*)g.psi_e[2][0])[o];
-          qe[5][e] = ((const T*)g.psi_e[2][1])[o];
+          const uint32_t o = fslab_off(g, 2, szz, i, j, kk);
+          qe[4][e] = *zp<T>(z.bc, z.pe[2] + (uint32_t)(0) * z.ps[2], o);
+          qe[5][e] = *zp<T>(z.bc, z.pe[2] + (uint32_t)(1) * z.ps[2], o);
         }
       }
     }
@@ -375,15 +387,15 @@ __global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, 
     const bool pf = row_ok && k + 1 <= kh;
     if (pf) {
       const uint32_t o = off + sz;
-      fld(nex, z.Ei[0] + o);
-      fld(ney, z.Ei[1] + o);
-      fld(nez, z.Ei[2] + o);
-      fld(nhx, z.Hi[0] + o);
-      fld(nhy, z.Hi[1] + o);
-      fld(nhz, z.Hi[2] + o);
+      fld(nex, zp<T>(z.bi, (uint32_t)(0) * z.cs, o));
+      fld(ney, zp<T>(z.bi, (uint32_t)(1) * z.cs, o));
+      fld(nez, zp<T>(z.bi, (uint32_t)(2) * z.cs, o));
+      fld(nhx, zp<T>(z.bi, (uint32_t)(3 + 0) * z.cs, o));
+      fld(nhy, zp<T>(z.bi, (uint32_t)(3 + 1) * z.cs, o));
+      fld(nhz, zp<T>(z.bi, (uint32_t)(3 + 2) * z.cs, o));
       if (first_row) {
-        fld(nexu, z.Ei[0] + o - sy);
-        fld(nezu, z.Ei[2] + o - sy);
+        fld(nexu, zp<T>(z.bi, (uint32_t)(0) * z.cs, o - sy));
+        fld(nezu, zp<T>(z.bi, (uint32_t)(2) * z.cs, o - sy));
       } else {
         fzero(nexu);
         fzero(nezu);
@@ -411,14 +423,14 @@ __global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, 
     __syncthreads();
     // (4) H(n+1/2) on plane k
     const bool store_h = own && k < ke;
-    const int szs = (pml && row_ok) ? fslab(k, nz, g.pml[4], g.pml[5]) : -1;
+    const int szs = (YZ && pml && row_ok) ? fslab(k, nz, g.pml[4], g.pml[5]) : -1;
     if (row_ok) {
       const bool pml_any = lane_pml_x || sys >= 0 || szs >= 0;
       T zbhk = 1, zchk = 0, zikhk = 1;
       if (szs >= 0) {
-        zbhk = zbh[k];
-        zchk = zch[k];
-        zikhk = zikh[k];
+        zbhk = *zp<T>(z.bc, z.prof[2] + 3u * z.pn[2], k);
+        zchk = *zp<T>(z.bc, z.prof[2] + 4u * z.pn[2], k);
+        zikhk = *zp<T>(z.bc, z.prof[2] + 5u * z.pn[2], k);
       }
       const T db = (T)g.db_const;
 #pragma unroll
@@ -436,34 +448,34 @@ __global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, 
         T f2 = (ex[e] - exj) * idy;
         if (pml_any && i < nx) {
           if (sys >= 0) {
-            const int64_t o = fslab_off(g, 1, sys, i, j, k);
+            const uint32_t o = fslab_off(g, 1, sys, i, j, k);
             const T p1 = ybh * qh[3][e] + ych * c1, p0 = ybh * qh[2][e] + ych * f2;
-            if (store_h && i < nx - 1) z.psh_out[1][1][o] = p1;
-            if (store_h && k < nz - 1) z.psh_out[1][0][o] = p0;
+            if (store_h && i < nx - 1) *zpw<T>(z.bc, z.pho[1] + (uint32_t)(1) * z.ps[1], o) = p1;
+            if (store_h && k < nz - 1) *zpw<T>(z.bc, z.pho[1] + (uint32_t)(0) * z.ps[1], o) = p0;
             c1 = c1 * yikh + p1;
             f2 = f2 * yikh + p0;
           }
           if (szs >= 0) {
-            const int64_t o = fslab_off(g, 2, szs, i, j, k);
+            const uint32_t o = fslab_off(g, 2, szs, i, j, k);
             const T p0 = zbhk * qh[4][e] + zchk * c2, p1 = zbhk * qh[5][e] + zchk * d1;
-            if (store_h && i < nx - 1) z.psh_out[2][0][o] = p0;
-            if (store_h && j < ny - 1) z.psh_out[2][1][o] = p1;
+            if (store_h && i < nx - 1) *zpw<T>(z.bc, z.pho[2] + (uint32_t)(0) * z.ps[2], o) = p0;
+            if (store_h && j < ny - 1) *zpw<T>(z.bc, z.pho[2] + (uint32_t)(1) * z.ps[2], o) = p1;
             c2 = c2 * zikhk + p0;
             d1 = d1 * zikhk + p1;
           }
           if (sxs[e] >= 0) {
-            const int64_t o = fslab_off(g, 0, sxs[e], i, j, k);
+            const uint32_t o = fslab_off(g, 0, sxs[e], i, j, k);
             const T p0 = xbh[e] * qh[0][e] + xch[e] * d2, p1 = xbh[e] * qh[1][e] + xch[e] * f1;
-            if (store_h && j < ny - 1) z.psh_out[0][0][o] = p0;
-            if (store_h && k < nz - 1) z.psh_out[0][1][o] = p1;
+            if (store_h && j < ny - 1) *zpw<T>(z.bc, z.pho[0] + (uint32_t)(0) * z.ps[0], o) = p0;
+            if (store_h && k < nz - 1) *zpw<T>(z.bc, z.pho[0] + (uint32_t)(1) * z.ps[0], o) = p1;
             d2 = d2 * xikh[e] + p0;
             f1 = f1 * xikh[e] + p1;
           }
         }
         if constexpr (CK == PM_COEFF_FULL) {
-          hx[e] = ((const T*)g.da[0])[off + e] * hxi[e] - ((const T*)g.db[0])[off + e] * (c1 - c2);
-          hy[e] = ((const T*)g.da[1])[off + e] * hyi[e] - ((const T*)g.db[1])[off + e] * (d1 - d2);
-          hz[e] = ((const T*)g.da[2])[off + e] * hzi[e] - ((const T*)g.db[2])[off + e] * (f1 - f2);
+          hx[e] = *zp<T>(z.bc, z.da[0], off + e) * hxi[e] - *zp<T>(z.bc, z.db[0], off + e) * (c1 - c2);
+          hy[e] = *zp<T>(z.bc, z.da[1], off + e) * hyi[e] - *zp<T>(z.bc, z.db[1], off + e) * (d1 - d2);
+          hz[e] = *zp<T>(z.bc, z.da[2], off + e) * hzi[e] - *zp<T>(z.bc, z.db[2], off + e) * (f1 - f2);
         } else {
           hx[e] = hxi[e] - db * (c1 - c2);
           hy[e] = hyi[e] - db * (d1 - d2);
@@ -474,9 +486,9 @@ __global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, 
         if (i >= nx || k >= nz - 1) hz[e] = 0;
       }
       if (store_h) {
-        fst(z.Ho[0] + off, hx);
-        fst(z.Ho[1] + off, hy);
-        fst(z.Ho[2] + off, hz);
+        fst(zpw<T>(z.bo, (uint32_t)(3 + 0) * z.cs, off), hx);
+        fst(zpw<T>(z.bo, (uint32_t)(3 + 1) * z.cs, off), hy);
+        fst(zpw<T>(z.bo, (uint32_t)(3 + 2) * z.cs, off), hz);
       }
     } else {
       fzero(hx);
@@ -498,9 +510,9 @@ __global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, 
       const bool pml_any = lane_pml_x || sys >= 0 || szz >= 0;
       T zbek = 1, zcek = 0, zikek = 1;
       if (szz >= 0) {
-        zbek = zbe[kk];
-        zcek = zce[kk];
-        zikek = zike[kk];
+        zbek = *zp<T>(z.bc, z.prof[2] + 0u * z.pn[2], kk);
+        zcek = *zp<T>(z.bc, z.prof[2] + 1u * z.pn[2], kk);
+        zikek = *zp<T>(z.bc, z.prof[2] + 2u * z.pn[2], kk);
       }
       const bool ade = g.n_ade && j >= g.ade_j0 && j < g.ade_j1 && kk >= g.ade_k0 &&
                        kk < g.ade_k1;
@@ -522,26 +534,26 @@ __global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, 
         const bool ezok = i < nx - 1 && j < ny - 1;
         if (pml_any) {
           if (sys >= 0) {
-            const int64_t oe = fslab_off(g, 1, sys, i, j, kk);
+            const uint32_t oe = fslab_off(g, 1, sys, i, j, kk);
             const T p1 = ybe * qe[3][e] + yce * a1, p0 = ybe * qe[2][e] + yce * c2;
-            if (exok) ((T*)g.psi_e[1][1])[oe] = p1;
-            if (ezok) ((T*)g.psi_e[1][0])[oe] = p0;
+            if (exok) *zpw<T>(z.bc, z.pe[1] + (uint32_t)(1) * z.ps[1], oe) = p1;
+            if (ezok) *zpw<T>(z.bc, z.pe[1] + (uint32_t)(0) * z.ps[1], oe) = p0;
             a1 = a1 * yike + p1;
             c2 = c2 * yike + p0;
           }
           if (szz >= 0) {
-            const int64_t oe = fslab_off(g, 2, szz, i, j, kk);
+            const uint32_t oe = fslab_off(g, 2, szz, i, j, kk);
             const T p0 = zbek * qe[4][e] + zcek * a2, p1 = zbek * qe[5][e] + zcek * b1;
-            if (exok) ((T*)g.psi_e[2][0])[oe] = p0;
-            if (eyok) ((T*)g.psi_e[2][1])[oe] = p1;
+            if (exok) *zpw<T>(z.bc, z.pe[2] + (uint32_t)(0) * z.ps[2], oe) = p0;
+            if (eyok) *zpw<T>(z.bc, z.pe[2] + (uint32_t)(1) * z.ps[2], oe) = p1;
             a2 = a2 * zikek + p0;
             b1 = b1 * zikek + p1;
           }
           if (sxs[e] >= 0) {
-            const int64_t oe = fslab_off(g, 0, sxs[e], i, j, kk);
+            const uint32_t oe = fslab_off(g, 0, sxs[e], i, j, kk);
             const T p0 = xbe[e] * qe[0][e] + xce[e] * b2, p1 = xbe[e] * qe[1][e] + xce[e] * c1;
-            if (eyok) ((T*)g.psi_e[0][0])[oe] = p0;
-            if (ezok) ((T*)g.psi_e[0][1])[oe] = p1;
+            if (eyok) *zpw<T>(z.bc, z.pe[0] + (uint32_t)(0) * z.ps[0], oe) = p0;
+            if (ezok) *zpw<T>(z.bc, z.pe[0] + (uint32_t)(1) * z.ps[0], oe) = p1;
             b2 = b2 * xike[e] + p0;
             c1 = c1 * xike[e] + p1;
           }
@@ -549,9 +561,9 @@ __global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, 
         const T cx = a1 - a2, cy = b1 - b2, cz = c1 - c2;
         T vx = exm[e], vy = eym[e], vz = ezm[e];
         if constexpr (CK == PM_COEFF_FULL) {
-          vx = ((const T*)g.ca[0])[o + e] * vx + cbx[e] * cx;
-          vy = ((const T*)g.ca[1])[o + e] * vy + cby[e] * cy;
-          vz = ((const T*)g.ca[2])[o + e] * vz + cbz[e] * cz;
+          vx = *zp<T>(z.bc, z.ca[0], o + e) * vx + cbx[e] * cx;
+          vy = *zp<T>(z.bc, z.ca[1], o + e) * vy + cby[e] * cy;
+          vz = *zp<T>(z.bc, z.ca[2], o + e) * vz + cbz[e] * cz;
         } else {
           vx = vx + cbx[e] * cx;
           vy = vy + cby[e] * cy;
@@ -575,9 +587,9 @@ __global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, 
         oy[e] = eyok ? vy : (T)0;
         oz[e] = ezok ? vz : (T)0;
       }
-      fst(z.Eo[0] + o, ox);
-      fst(z.Eo[1] + o, oy);
-      fst(z.Eo[2] + o, oz);
+      fst(zpw<T>(z.bo, (uint32_t)(0) * z.cs, o), ox);
+      fst(zpw<T>(z.bo, (uint32_t)(1) * z.cs, o), oy);
+      fst(zpw<T>(z.bo, (uint32_t)(2) * z.cs, o), oz);
     }
     // (7) rotate planes, adopt the prefetched plane
 #pragma unroll
@@ -606,7 +618,7 @@ __global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, 
     const uint32_t o = rowoff + (uint32_t)kk * sz;
     const int p = kk & 1;
     T cbz[VEC];
-    fcb<T, CK, VEC>(g, 2, o, cbz);
+    fcb<T, CK, VEC>(g, z, 2, o, cbz);
     const bool ade = g.n_ade && j >= g.ade_j0 && j < g.ade_j1 && kk >= g.ade_k0 &&
                      kk < g.ade_k1;
     T oz[VEC], zero[VEC];
@@ -619,12 +631,12 @@ __global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, 
       T c2 = (lh[p][0][w + 1][i] - hxp[e]) * idy;
       const bool ezok = i < nx - 1 && j < ny - 1;
       if (ezok && pml) {
-        if (sys >= 0) c2 = fcpml_e(g, c2, 1, 0, j, sys, i, j, kk);
-        if (sxs[e] >= 0) c1 = fcpml_e(g, c1, 0, 1, i, sxs[e], i, j, kk);
+        if (sys >= 0) c2 = fcpml_e<T>(g, z, c2, 1, 0, j, sys, i, j, kk);
+        if (sxs[e] >= 0) c1 = fcpml_e<T>(g, z, c1, 0, 1, i, sxs[e], i, j, kk);
       }
       const T cz = c1 - c2;
       T vz;
-      if constexpr (CK == PM_COEFF_FULL) vz = ((const T*)g.ca[2])[o + e] * ezm[e] + cbz[e] * cz;
+      if constexpr (CK == PM_COEFF_FULL) vz = *zp<T>(z.bc, z.ca[2], o + e) * ezm[e] + cbz[e] * cz;
       else vz = ezm[e] + cbz[e] * cz;
       if (ade && ezok) {
         const AdeOut<T> r = fade<T>(g.ade, g.n_ade, 2, i, j, kk, cz, ezm[e]);
@@ -632,9 +644,9 @@ __global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, 
       }
       oz[e] = ezok ? vz : (T)0;
     }
-    fst(z.Eo[0] + o, zero);
-    fst(z.Eo[1] + o, zero);
-    fst(z.Eo[2] + o, oz);
+    fst(zpw<T>(z.bo, (uint32_t)(0) * z.cs, o), zero);
+    fst(zpw<T>(z.bo, (uint32_t)(1) * z.cs, o), zero);
+    fst(zpw<T>(z.bo, (uint32_t)(2) * z.cs, o), oz);
   }
 }
 
@@ -644,14 +656,33 @@ constexpr bool fused_fits() {
 }
 
 template <typename T, int CK, int VEC, int W, int TY>
-int launch_fused_one(const KGrid& g, const Fz<T>& z, int kchunk, int order, hipStream_t s) {
+int launch_fused_one(const KGrid& g, const KFz& z, int kchunk, int order, hipStream_t s) {
   if constexpr (fused_fits<T, VEC, W, TY>()) {
     const int ntile_y = (g.ny + TY - 1) / TY;
     const int nchunk = (g.nz + kchunk - 1) / kchunk;
-    const int nb = ntile_y * nchunk;
-    const int nlaunch = ((nb + 7) / 8) * 8;
-    hipLaunchKernelGGL((k_yee_fused<T, CK, VEC, W, TY>), dim3(nlaunch), dim3(64 * W * (TY + 1)),
-                       0, s, g, z, kchunk, ntile_y, nchunk, nb, order);
+    // interior box: a tile computes H on rows j0 .. j0+TY and E on rows
+    // j0 .. j0+TY-1, planes kb .. ke (H) and kb .. ke-1 (E)
+    TileBox bx{0, ntile_y, 0, nchunk};
+    if (g.has_pml) {
+      const int ylo = g.pml[2], yhi = g.ny - g.pml[3];  // rows [ylo, yhi) are inside
+      const int zlo = g.pml[4], zhi = g.nz - g.pml[5];
+      bx.ty0 = (ylo + TY - 1) / TY;
+      bx.ty1 = std::max(bx.ty0, (yhi - 1 - TY) / TY + 1);
+      if ((bx.ty1 - 1) * TY + TY > yhi - 1) bx.ty1 = bx.ty0;
+      bx.kc0 = (zlo + kchunk - 1) / kchunk;
+      bx.kc1 = std::max(bx.kc0, (zhi - 1) / kchunk);
+      if (bx.kc1 > bx.kc0 && (bx.kc1 - 1) * kchunk + kchunk > zhi - 1) bx.kc1 = bx.kc0;
+      if (bx.ty1 <= bx.ty0 || bx.kc1 <= bx.kc0) bx = TileBox{0, 0, 0, 0};
+    }
+    const int n_in = (bx.ty1 - bx.ty0) * (bx.kc1 - bx.kc0);
+    const int n_out = ntile_y * nchunk - n_in;
+    const dim3 block(64 * W * (TY + 1));
+    if (n_in > 0)
+      hipLaunchKernelGGL((k_yee_fused<T, CK, VEC, W, TY, false>), dim3(((n_in + 7) / 8) * 8),
+                         block, 0, s, g, z, kchunk, ntile_y, nchunk, n_in, order, bx);
+    if (n_out > 0)
+      hipLaunchKernelGGL((k_yee_fused<T, CK, VEC, W, TY, true>), dim3(((n_out + 7) / 8) * 8),
+                         block, 0, s, g, z, kchunk, ntile_y, nchunk, n_out, order, bx);
     return 0;
   } else {
     return -1;
@@ -659,7 +690,7 @@ int launch_fused_one(const KGrid& g, const Fz<T>& z, int kchunk, int order, hipS
 }
 
 template <typename T, int CK, int VEC, int W>
-int launch_fused_w(const KGrid& g, const Fz<T>& z, int ty, int kchunk, int order,
+int launch_fused_w(const KGrid& g, const KFz& z, int ty, int kchunk, int order,
                    hipStream_t s) {
   if (ty == 2) return launch_fused_one<T, CK, VEC, W, 2>(g, z, kchunk, order, s);
   if (ty == 4) return launch_fused_one<T, CK, VEC, W, 4>(g, z, kchunk, order, s);
@@ -668,7 +699,7 @@ int launch_fused_w(const KGrid& g, const Fz<T>& z, int ty, int kchunk, int order
 
 // (vec, w) pairs with 64*vec*w = row width 128, 256 or 512
 template <typename T, int CK>
-int launch_fused_ck(const KGrid& g, const Fz<T>& z, int vec, int w, int ty, int kchunk,
+int launch_fused_ck(const KGrid& g, const KFz& z, int vec, int w, int ty, int kchunk,
                     int order, hipStream_t s) {
   if (vec == 1 && w == 2) return launch_fused_w<T, CK, 1, 2>(g, z, ty, kchunk, order, s);
   if (vec == 2 && w == 1) return launch_fused_w<T, CK, 2, 1>(g, z, ty, kchunk, order, s);
@@ -680,20 +711,111 @@ int launch_fused_ck(const KGrid& g, const Fz<T>& z, int vec, int w, int ty, int 
   return -1;
 }
 
-template <typename T>
-Fz<T> make_fz(const FusedBufs& fb) {
-  Fz<T> z{};
-  for (int a = 0; a < 3; ++a) {
-    z.Ei[a] = (const T*)fb.Ei[a];
-    z.Hi[a] = (const T*)fb.Hi[a];
-    z.Eo[a] = (T*)fb.Eo[a];
-    z.Ho[a] = (T*)fb.Ho[a];
-    for (int q = 0; q < 2; ++q) {
-      z.psh_in[a][q] = (const T*)fb.psh_in[a][q];
-      z.psh_out[a][q] = (T*)fb.psh_out[a][q];
-    }
+// Offsets of the arrays from the lowest address of each group (field set
+// read, field set written, everything else).  The kernel derives most
+// offsets from a few strides, so it needs the packed layout the engine
+// allocates: the six components of a field set at a constant stride, the
+// three Cb arrays at a constant stride, the two CPML memory arrays of an
+// axis back to back, the six profiles of an axis back to back.  False when
+// the plan is laid out otherwise or a group spans 4 GiB (the caller then
+// steps unfused).
+inline bool make_kfz_impl(const KGrid& g, const FusedBufs& fb, int esz, KFz* out,
+                          const char** why) {
+  KFz z{};
+  auto u = [](const void* p) { return (uintptr_t)p; };
+#define PM_KFZ_FAIL(msg) \
+  do {                   \
+    *why = msg;          \
+    return false;        \
+  } while (0)
+  const uint64_t field = (uint64_t)g.sz * (uint64_t)g.nz * (uint64_t)esz;
+  const int n[3] = {g.nx, g.ny, g.nz};
+  // field sets: Ex Ey Ez Hx Hy Hz at a constant stride
+  const void* fin[6] = {fb.Ei[0], fb.Ei[1], fb.Ei[2], fb.Hi[0], fb.Hi[1], fb.Hi[2]};
+  const void* fout[6] = {fb.Eo[0], fb.Eo[1], fb.Eo[2], fb.Ho[0], fb.Ho[1], fb.Ho[2]};
+  for (int c = 0; c < 6; ++c)
+    if (!fin[c] || !fout[c]) PM_KFZ_FAIL("missing field array");
+  const intptr_t cs = (intptr_t)(u(fin[1]) - u(fin[0]));
+  if (cs < (intptr_t)field || 6 * (uint64_t)cs >= (1ull << 32))
+    PM_KFZ_FAIL("field set is not packed (component stride)");
+  for (int c = 1; c < 6; ++c) {
+    if ((intptr_t)(u(fin[c]) - u(fin[c - 1])) != cs) PM_KFZ_FAIL("input field set not packed");
+    if ((intptr_t)(u(fout[c]) - u(fout[c - 1])) != cs) PM_KFZ_FAIL("output field set not packed");
   }
-  return z;
+  z.bi = (const char*)fin[0];
+  z.bo = (const char*)fout[0];
+  z.cs = (uint32_t)cs;
+  // everything else relative to the lowest of it
+  const void* rest[64];
+  int nr = 0;
+  auto add = [&](const void* p) {
+    if (p) rest[nr++] = p;
+  };
+  for (int a = 0; a < 3; ++a) {
+    add(g.cb[a]);
+    add(g.ca[a]);
+    add(g.da[a]);
+    add(g.db[a]);
+    add(g.pml_be[a]);
+    add(g.psi_e[a][0]);
+    add(fb.psh_in[a][0]);
+    add(fb.psh_out[a][0]);
+  }
+  if (nr == 0) PM_KFZ_FAIL("no coefficient arrays");
+  uintptr_t lo = ~(uintptr_t)0;
+  for (int t = 0; t < nr; ++t) lo = std::min(lo, u(rest[t]));
+  auto off = [&](const void* p, uint32_t* slot) {
+    if (!p) {
+      *slot = 0;
+      return true;
+    }
+    const uint64_t d = (uint64_t)(u(p) - lo);
+    if (d + 2 * field >= (1ull << 32)) return false;
+    *slot = (uint32_t)d;
+    return true;
+  };
+  z.bc = (const char*)lo;
+  if (!g.cb[0] || !g.cb[1] || !g.cb[2]) PM_KFZ_FAIL("missing Cb");
+  const intptr_t ccs = (intptr_t)(u(g.cb[1]) - u(g.cb[0]));
+  if (ccs <= 0 || (intptr_t)(u(g.cb[2]) - u(g.cb[1])) != ccs) PM_KFZ_FAIL("Cb not packed");
+  z.ccs = (uint32_t)ccs;
+  if (!off(g.cb[0], &z.cb)) PM_KFZ_FAIL("Cb beyond 4 GiB");
+  for (int a = 0; a < 3; ++a) {
+    if (!off(g.ca[a], &z.ca[a]) || !off(g.da[a], &z.da[a]) || !off(g.db[a], &z.db[a]))
+      PM_KFZ_FAIL("Ca/Da/Db beyond 4 GiB");
+    if (g.pml[2 * a] + g.pml[2 * a + 1] == 0) continue;
+    // CPML memory pairs back to back, profiles back to back
+    const uint64_t S = (uint64_t)(g.pml[2 * a] + g.pml[2 * a + 1]);
+    const uint64_t ps = S * (uint64_t)g.nx * g.ny * g.nz / (uint64_t)n[a] * (uint64_t)esz;
+    z.ps[a] = (uint32_t)ps;
+    const void* pairs[3][2] = {{g.psi_e[a][0], g.psi_e[a][1]},
+                               {fb.psh_in[a][0], fb.psh_in[a][1]},
+                               {fb.psh_out[a][0], fb.psh_out[a][1]}};
+    for (auto& pr : pairs)
+      if (!pr[0] || !pr[1] || u(pr[1]) - u(pr[0]) != ps) PM_KFZ_FAIL("CPML memory pair not packed");
+    if (!off(g.psi_e[a][0], &z.pe[a]) || !off(fb.psh_in[a][0], &z.phi[a]) ||
+        !off(fb.psh_out[a][0], &z.pho[a]))
+      PM_KFZ_FAIL("CPML memory beyond 4 GiB");
+    const uintptr_t pn = (uintptr_t)n[a] * (uintptr_t)esz;
+    const void* pr6[6] = {g.pml_be[a], g.pml_ce[a], g.pml_ike[a],
+                          g.pml_bh[a], g.pml_ch[a], g.pml_ikh[a]};
+    for (int t = 0; t < 6; ++t)
+      if (!pr6[t] || u(pr6[t]) != u(pr6[0]) + (uintptr_t)t * pn) PM_KFZ_FAIL("profiles not packed");
+    z.pn[a] = (uint32_t)pn;
+    if (!off(g.pml_be[a], &z.prof[a])) PM_KFZ_FAIL("profiles beyond 4 GiB");
+  }
+  *out = z;
+  return true;
+#undef PM_KFZ_FAIL
+}
+
+// make_kfz_impl; PRISMO_AMD_DEBUG_FUSED=1 prints why a plan is not laid out
+// for the fused kernel.
+inline bool make_kfz(const KGrid& g, const FusedBufs& fb, int esz, KFz* out) {
+  const char* why = "";
+  const bool ok = make_kfz_impl(g, fb, esz, out, &why);
+  if (!ok && std::getenv("PRISMO_AMD_DEBUG_FUSED")) std::fprintf(stderr, "[pmfdtd] fused layout: %s\n", why);
+  return ok;
 }
 
 }  // namespace pm

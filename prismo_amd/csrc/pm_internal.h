@@ -134,6 +134,9 @@ int launch_yee_fused_f32(const KGrid& g, int ck, const FusedBufs& fb, int vec, i
 int launch_yee_fused_f64(const KGrid& g, int ck, const FusedBufs& fb, int vec, int w, int ty,
                          int kchunk, int order, hipStream_t s);
 bool fused_supported(int precision, int vec, int w, int ty);
+// The plan's arrays are laid out as the fused kernel addresses them
+// (packed field sets, Cb, CPML memory pairs and profiles within 4 GiB).
+bool fused_addressable(const KGrid& g, const FusedBufs& fb, int esz);
 void launch_sources(const KAux& a, int64_t step, hipStream_t s);
 void launch_sample(const KAux& a, int64_t step, hipStream_t s);
 void launch_dft(const KDft& d, const void* X, int ncols, const void* W_re, const void* W_im,

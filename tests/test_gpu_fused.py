@@ -65,7 +65,7 @@ def _fields(sim):
 def test_fused_fp64_matches_oracle(mi355x, shape, steps):
     sim, eps = _sim(*shape, t=5, nsrc=2)
     eng = sim.solver.updater.engine_for(sim.fields)
-    assert eng.fused, "shape should take the fused path"
+    assert eng.fused and eng.ctx.fused_active(), "shape should take the fused path"
     sim.run(steps * sim.dt * 0.9999)
     o1 = _oracle(sim, eps, 5)
     _drive(o1, sim, steps)
@@ -76,10 +76,11 @@ def test_fused_fp64_matches_oracle(mi355x, shape, steps):
 
 
 def test_fused_tilings_agree(mi355x):
-    """Tile shape and order never change the result: bit-identical for one
-    vector width (same instruction stream per cell); across widths the
-    compiler may fuse different multiply-adds, so <= 1e-13."""
-    by_vec = {}
+    """Tile shape and order never change the result beyond rounding: a tiling
+    decides whether a cell is stepped by the interior instance of the kernel
+    or the CPML-slab one, and the vector width changes the instruction
+    stream, so the compiler may fuse different multiply-adds (<= 1e-13)."""
+    ref = None
     for ty, kc, vec, order in ((4, 16, 1, 0), (2, 1, 1, 0), (4, 5, 1, 2), (2, 3, 2, 1),
                                (4, 7, 2, 0), (2, 16, 2, 2)):
         sim, _ = _sim(128, 22, 19, t=4)
@@ -88,10 +89,9 @@ def test_fused_tilings_agree(mi355x):
         assert eng.ctx.fused_active()
         sim.run(24 * sim.dt * 0.9999)
         f = _all(_fields(sim))
-        if vec in by_vec:
-            assert np.array_equal(f.view(np.uint64), by_vec[vec].view(np.uint64)), (ty, kc, vec, order)
-        by_vec.setdefault(vec, f)
-    assert rel_l2(by_vec[1], by_vec[2]) <= 1e-13
+        if ref is None:
+            ref = f
+        assert rel_l2(f, ref) <= 1e-13, (ty, kc, vec, order)
 
 
 def test_fused_equals_unfused(mi355x):
