@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <type_traits>
 
 #include "pm_internal.h"
 
@@ -153,23 +154,29 @@ __device__ __forceinline__ void fzero(T (&d)[VEC]) {
   for (int e = 0; e < VEC; ++e) d[e] = (T)0;
 }
 
+// Cb as stored (FP16 or the field type), loaded early and converted where
+// it is used, so the load's latency is not waited for right away.
+template <typename T, int CK>
+using CbRaw = typename std::conditional<CK == PM_COEFF_CB_F16, _Float16, T>::type;
+
 template <typename T, int CK, int VEC>
-__device__ __forceinline__ void fcb(const KGrid& g, const KFz& z, int c, uint32_t off,
-                                    T (&cb)[VEC]) {
-  if constexpr (CK == PM_COEFF_CB_F16) {
-    const _Float16* h = (const _Float16*)(z.bc + (z.cb + (uint32_t)c * z.ccs + off * 2u));
-    const T s = (T)g.cb_scale;
-    if constexpr (VEC == 1) {
-      cb[0] = s * (T)h[0];
-    } else {
-      using HV = _Float16 __attribute__((ext_vector_type(VEC)));
-      const HV v = *(const HV*)h;
-#pragma unroll
-      for (int e = 0; e < VEC; ++e) cb[e] = s * (T)v[e];
-    }
+__device__ __forceinline__ void fcb(const KFz& z, int c, uint32_t off, CbRaw<T, CK> (&cb)[VEC]) {
+  using R = CbRaw<T, CK>;
+  const R* h = (const R*)(z.bc + (z.cb + (uint32_t)c * z.ccs + off * (uint32_t)sizeof(R)));
+  if constexpr (VEC == 1) {
+    cb[0] = h[0];
   } else {
-    fld<T, VEC>(cb, zp<T>(z.bc, z.cb + (uint32_t)c * z.ccs, off));
+    using HV = R __attribute__((ext_vector_type(VEC)));
+    const HV v = *(const HV*)h;
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) cb[e] = v[e];
   }
+}
+
+template <typename T, int CK>
+__device__ __forceinline__ T fcbv(const KGrid& g, CbRaw<T, CK> r) {
+  if constexpr (CK == PM_COEFF_CB_F16) return (T)g.cb_scale * (T)r;
+  else return r;
 }
 
 // Tiles whose rows and planes all lie outside the y and z CPML slabs form a
@@ -349,13 +356,13 @@ __global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, 
     // (1) what this iteration's E update needs, issued ahead of the next
     //     plane's prefetch: vector memory counters retire in order, so a
     //     later wait on these does not wait on the prefetch
-    T cbx[VEC], cby[VEC], cbz[VEC];
+    CbRaw<T, CK> cbx[VEC], cby[VEC], cbz[VEC];
     T qe[6][VEC];
     const int szz = (YZ && pml && do_e) ? fslab(kk, nz, g.pml[4], g.pml[5]) : -1;
     if (do_e) {
-      fcb<T, CK, VEC>(g, z, 0, off - sz, cbx);
-      fcb<T, CK, VEC>(g, z, 1, off - sz, cby);
-      fcb<T, CK, VEC>(g, z, 2, off - sz, cbz);
+      fcb<T, CK, VEC>(z, 0, off - sz, cbx);
+      fcb<T, CK, VEC>(z, 1, off - sz, cby);
+      fcb<T, CK, VEC>(z, 2, off - sz, cbz);
     }
     if (pml) {
 #pragma unroll
@@ -561,13 +568,13 @@ __global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, 
         const T cx = a1 - a2, cy = b1 - b2, cz = c1 - c2;
         T vx = exm[e], vy = eym[e], vz = ezm[e];
         if constexpr (CK == PM_COEFF_FULL) {
-          vx = *zp<T>(z.bc, z.ca[0], o + e) * vx + cbx[e] * cx;
-          vy = *zp<T>(z.bc, z.ca[1], o + e) * vy + cby[e] * cy;
-          vz = *zp<T>(z.bc, z.ca[2], o + e) * vz + cbz[e] * cz;
+          vx = *zp<T>(z.bc, z.ca[0], o + e) * vx + fcbv<T, CK>(g, cbx[e]) * cx;
+          vy = *zp<T>(z.bc, z.ca[1], o + e) * vy + fcbv<T, CK>(g, cby[e]) * cy;
+          vz = *zp<T>(z.bc, z.ca[2], o + e) * vz + fcbv<T, CK>(g, cbz[e]) * cz;
         } else {
-          vx = vx + cbx[e] * cx;
-          vy = vy + cby[e] * cy;
-          vz = vz + cbz[e] * cz;
+          vx = vx + fcbv<T, CK>(g, cbx[e]) * cx;
+          vy = vy + fcbv<T, CK>(g, cby[e]) * cy;
+          vz = vz + fcbv<T, CK>(g, cbz[e]) * cz;
         }
         if (ade) {
           if (exok) {
@@ -617,8 +624,8 @@ __global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, 
     const int kk = nz - 1;
     const uint32_t o = rowoff + (uint32_t)kk * sz;
     const int p = kk & 1;
-    T cbz[VEC];
-    fcb<T, CK, VEC>(g, z, 2, o, cbz);
+    CbRaw<T, CK> cbz[VEC];
+    fcb<T, CK, VEC>(z, 2, o, cbz);
     const bool ade = g.n_ade && j >= g.ade_j0 && j < g.ade_j1 && kk >= g.ade_k0 &&
                      kk < g.ade_k1;
     T oz[VEC], zero[VEC];
@@ -636,8 +643,10 @@ __global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, 
       }
       const T cz = c1 - c2;
       T vz;
-      if constexpr (CK == PM_COEFF_FULL) vz = *zp<T>(z.bc, z.ca[2], o + e) * ezm[e] + cbz[e] * cz;
-      else vz = ezm[e] + cbz[e] * cz;
+      if constexpr (CK == PM_COEFF_FULL)
+        vz = *zp<T>(z.bc, z.ca[2], o + e) * ezm[e] + fcbv<T, CK>(g, cbz[e]) * cz;
+      else
+        vz = ezm[e] + fcbv<T, CK>(g, cbz[e]) * cz;
       if (ade && ezok) {
         const AdeOut<T> r = fade<T>(g.ade, g.n_ade, 2, i, j, kk, cz, ezm[e]);
         if (r.hit) vz = r.e;
