@@ -188,16 +188,23 @@ struct TileBox {
   int ty0, ty1, kc0, kc1;
 };
 
-template <typename T, int CK, int VEC, int W, int TY, bool YZ>
-__global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, const KFz z,
-                                                                 int kchunk, int ntile_y,
-                                                                 int nchunk, int nblocks,
-                                                                 int order, TileBox bx) {
-  constexpr int RW = 64 * VEC * W;
-  __shared__ T le[3][TY + 1][RW];     // E(n) of plane k
-  __shared__ T lh[2][3][TY + 1][RW];  // H(n+1/2) of planes with parity 0 / 1
+template <typename T, int VEC, int W, int TY>
+struct FusedLds {
+  static constexpr int RW = 64 * VEC * W;
+  T le[3][TY + 1][RW];     // E(n) of plane k
+  T lh[2][3][TY + 1][RW];  // H(n+1/2) of planes with parity 0 / 1
+};
 
-  const int b = blockIdx.x;
+// One tile (TY rows x one z-chunk); b = the workgroup's index within the
+// launch segment of its kind (slab tiles or interior tiles).
+template <typename T, int CK, int VEC, int W, int TY, bool YZ>
+__device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kchunk,
+                                           int ntile_y, int nchunk, int nblocks, int order,
+                                           const TileBox& bx, int b,
+                                           FusedLds<T, VEC, W, TY>& lds) {
+  constexpr int RW = 64 * VEC * W;
+  auto& le = lds.le;
+  auto& lh = lds.lh;
   const int per = (nblocks + 7) / 8;
   // orders 0/1: XCD b % 8 runs a contiguous range of tiles (halo reuse in its
   // L2); order 2: hardware round-robin (slab tiles spread over all XCDs)
@@ -247,7 +254,6 @@ __global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, 
   const T idx = (T)g.idx, idy = (T)g.idy, idz = (T)g.idz;
   // 32-bit element offsets (the host keeps every field array below 4 GB)
   const uint32_t sy = (uint32_t)g.sy, sz = (uint32_t)g.sz;
-  const uint32_t rowoff = (uint32_t)j * sy + (uint32_t)x0;
   const bool first_row = (w == 0) && row_ok && j > 0;
   const bool pml = g.has_pml;
 
@@ -283,219 +289,181 @@ __global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, 
     yike = *zp<T>(z.bc, z.prof[1] + 2u * z.pn[1], j);
   }
 
-  // H-side CPML memory of one plane: [x0 x1 y0 y1 z0 z1] per cell
-  auto load_qh = [&](int k, T (&q)[6][VEC]) {
-    const int sk = (YZ && pml && row_ok) ? fslab(k, nz, g.pml[4], g.pml[5]) : -1;
+  // Every global load in the z-march is issued on every path, from a
+  // clamped in-range position where its value goes unused (rows past ny,
+  // the plane after the chunk, lanes outside a CPML slab).  The compiler
+  // then counts the same number of loads on every path, so each memory wait
+  // covers only the loads it needs and never the plane still in flight.
+  // Planes alternate between two register sets (the loop is unrolled by
+  // two), so no in-flight register is ever copied.
+  const int jr = min(j, ny - 1);
+  const uint32_t rowoff = (uint32_t)jr * sy + (uint32_t)x0;
+  const uint32_t upoff = (uint32_t)(first_row ? j - 1 : jr) * sy + (uint32_t)x0;
+  int sxc[VEC];
+#pragma unroll
+  for (int e = 0; e < VEC; ++e) sxc[e] = max(sxs[e], 0);
+  const int syc = max(sys, 0);
+
+  // CPML memory of one plane, [x0 x1 y0 y1 z0 z1] per cell; base = the
+  // first array of each axis' pair (an axis without CPML points at a large
+  // array the host picks, so the clamped load stays in bounds)
+  auto load_psi = [&](const uint32_t(&base)[3], int kq, T(&q)[6][VEC])
+                      __attribute__((always_inline)) {
+    const int skc = YZ ? max(fslab(kq, nz, g.pml[4], g.pml[5]), 0) : 0;
 #pragma unroll
     for (int e = 0; e < VEC; ++e) {
-      const int i = x0 + e;
+      const int ic = min(x0 + e, nx - 1);
+      const uint32_t ox = fslab_off(g, 0, sxc[e], ic, jr, kq);
+      q[0][e] = *zp<T>(z.bc, base[0], ox);
+      q[1][e] = *zp<T>(z.bc, base[0] + z.ps[0], ox);
+      if constexpr (YZ) {
+        const uint32_t oy = fslab_off(g, 1, syc, ic, jr, kq);
+        q[2][e] = *zp<T>(z.bc, base[1], oy);
+        q[3][e] = *zp<T>(z.bc, base[1] + z.ps[1], oy);
+        const uint32_t oz = fslab_off(g, 2, skc, ic, jr, kq);
+        q[4][e] = *zp<T>(z.bc, base[2], oz);
+        q[5][e] = *zp<T>(z.bc, base[2] + z.ps[2], oz);
+      } else {
 #pragma unroll
-      for (int t = 0; t < 6; ++t) q[t][e] = (T)0;
-      if (sxs[e] >= 0) {
-        const uint32_t o = fslab_off(g, 0, sxs[e], i, j, k);
-        q[0][e] = *zp<T>(z.bc, z.phi[0] + (uint32_t)(0) * z.ps[0], o);
-        q[1][e] = *zp<T>(z.bc, z.phi[0] + (uint32_t)(1) * z.ps[0], o);
-      }
-      if (sys >= 0 && i < nx) {
-        const uint32_t o = fslab_off(g, 1, sys, i, j, k);
-        q[2][e] = *zp<T>(z.bc, z.phi[1] + (uint32_t)(0) * z.ps[1], o);
-        q[3][e] = *zp<T>(z.bc, z.phi[1] + (uint32_t)(1) * z.ps[1], o);
-      }
-      if (sk >= 0 && i < nx) {
-        const uint32_t o = fslab_off(g, 2, sk, i, j, k);
-        q[4][e] = *zp<T>(z.bc, z.phi[2] + (uint32_t)(0) * z.ps[2], o);
-        q[5][e] = *zp<T>(z.bc, z.phi[2] + (uint32_t)(1) * z.ps[2], o);
+        for (int t = 2; t < 6; ++t) q[t][e] = (T)0;
       }
     }
   };
 
-  T ex[VEC], ey[VEC], ez[VEC], exm[VEC], eym[VEC], ezm[VEC];
-  T hx[VEC], hy[VEC], hz[VEC], hxp[VEC], hyp[VEC], hzp[VEC];
-  T hxi[VEC], hyi[VEC], hzi[VEC];
-  T exu[VEC], ezu[VEC];  // row j-1 of plane k (first row only)
-  T qh[6][VEC];
+  struct PlaneIn {
+    T ex[VEC], ey[VEC], ez[VEC];  // E(n) of the plane
+    T hx[VEC], hy[VEC], hz[VEC];  // H(n-1/2)
+    T exu[VEC], ezu[VEC];         // row j-1 (first row of the tile only)
+    T q[6][VEC];                  // H-side CPML memory
+  };
+  auto load_plane = [&](int kq, PlaneIn& P) __attribute__((always_inline)) {
+    const uint32_t o = rowoff + (uint32_t)kq * sz;
+    fld(P.ex, zp<T>(z.bi, 0u, o));
+    fld(P.ey, zp<T>(z.bi, z.cs, o));
+    fld(P.ez, zp<T>(z.bi, 2u * z.cs, o));
+    fld(P.hx, zp<T>(z.bi, 3u * z.cs, o));
+    fld(P.hy, zp<T>(z.bi, 4u * z.cs, o));
+    fld(P.hz, zp<T>(z.bi, 5u * z.cs, o));
+    const uint32_t u = upoff + (uint32_t)kq * sz;
+    fld(P.exu, zp<T>(z.bi, 0u, u));
+    fld(P.ezu, zp<T>(z.bi, 2u * z.cs, u));
+    load_psi(z.phi, kq, P.q);
+  };
+
+  T exm[VEC], eym[VEC], ezm[VEC];  // E(n) of plane k-1
+  T hxp[VEC], hyp[VEC], hzp[VEC];  // H(n+1/2) of plane k-1
   fzero(exm);
   fzero(eym);
   fzero(ezm);
   fzero(hxp);
   fzero(hyp);
   fzero(hzp);
-  fzero(exu);
-  fzero(ezu);
   if (row_ok && kb > 0) {
     const uint32_t o = rowoff + (uint32_t)(kb - 1) * sz;
-    fld(exm, zp<T>(z.bi, (uint32_t)(0) * z.cs, o));
-    fld(eym, zp<T>(z.bi, (uint32_t)(1) * z.cs, o));
+    fld(exm, zp<T>(z.bi, 0u, o));
+    fld(eym, zp<T>(z.bi, z.cs, o));
   }
-  if (row_ok) {
-    const uint32_t o = rowoff + (uint32_t)kb * sz;
-    fld(ex, zp<T>(z.bi, (uint32_t)(0) * z.cs, o));
-    fld(ey, zp<T>(z.bi, (uint32_t)(1) * z.cs, o));
-    fld(ez, zp<T>(z.bi, (uint32_t)(2) * z.cs, o));
-    fld(hxi, zp<T>(z.bi, (uint32_t)(3 + 0) * z.cs, o));
-    fld(hyi, zp<T>(z.bi, (uint32_t)(3 + 1) * z.cs, o));
-    fld(hzi, zp<T>(z.bi, (uint32_t)(3 + 2) * z.cs, o));
-    if (first_row) {
-      fld(exu, zp<T>(z.bi, (uint32_t)(0) * z.cs, o - sy));
-      fld(ezu, zp<T>(z.bi, (uint32_t)(2) * z.cs, o - sy));
-    }
-  } else {
-    fzero(ex);
-    fzero(ey);
-    fzero(ez);
-    fzero(hxi);
-    fzero(hyi);
-    fzero(hzi);
-  }
-  if (pml) load_qh(kb, qh);
 
-  for (int k = kb; k <= kh; ++k) {
+  auto step = [&](const int k, const PlaneIn& cur, PlaneIn& nxt) __attribute__((always_inline)) {
     const uint32_t off = rowoff + (uint32_t)k * sz;
     const bool do_e = k > kb && own;
     const int kk = k - 1;
+    const int kq = k > kb ? kk : k;  // where the E-side loads read
     // (1) what this iteration's E update needs, issued ahead of the next
-    //     plane's prefetch: vector memory counters retire in order, so a
-    //     later wait on these does not wait on the prefetch
+    //     plane's prefetch: vector memory counters retire in order, so the
+    //     wait for these does not wait for the prefetch
     CbRaw<T, CK> cbx[VEC], cby[VEC], cbz[VEC];
+    const uint32_t oq = rowoff + (uint32_t)kq * sz;
+    fcb<T, CK, VEC>(z, 0, oq, cbx);
+    fcb<T, CK, VEC>(z, 1, oq, cby);
+    fcb<T, CK, VEC>(z, 2, oq, cbz);
     T qe[6][VEC];
-    const int szz = (YZ && pml && do_e) ? fslab(kk, nz, g.pml[4], g.pml[5]) : -1;
-    if (do_e) {
-      fcb<T, CK, VEC>(z, 0, off - sz, cbx);
-      fcb<T, CK, VEC>(z, 1, off - sz, cby);
-      fcb<T, CK, VEC>(z, 2, off - sz, cbz);
-    }
-    if (pml) {
+    load_psi(z.pe, kq, qe);
+    // z profiles of plane k (H side) and of the E plane (slab tiles only)
+    T zh[3] = {1, 0, 1}, ze[3] = {1, 0, 1};
+    if constexpr (YZ) {
 #pragma unroll
-      for (int e = 0; e < VEC; ++e) {
-        const int i = x0 + e;
-#pragma unroll
-        for (int t = 0; t < 6; ++t) qe[t][e] = (T)0;
-        if (!do_e || i >= nx) continue;
-        if (sxs[e] >= 0) {
-          const uint32_t o = fslab_off(g, 0, sxs[e], i, j, kk);
-          qe[0][e] = *zp<T>(z.bc, z.pe[0] + (uint32_t)(0) * z.ps[0], o);
-          qe[1][e] = *zp<T>(z.bc, z.pe[0] + (uint32_t)(1) * z.ps[0], o);
-        }
-        if (sys >= 0) {
-          const uint32_t o = fslab_off(g, 1, sys, i, j, kk);
-          qe[2][e] = *zp<T>(z.bc, z.pe[1] + (uint32_t)(0) * z.ps[1], o);
-          qe[3][e] = *zp<T>(z.bc, z.pe[1] + (uint32_t)(1) * z.ps[1], o);
-        }
-        if (szz >= 0) {
-          const uint32_t o = fslab_off(g, 2, szz, i, j, kk);
-          qe[4][e] = *zp<T>(z.bc, z.pe[2] + (uint32_t)(0) * z.ps[2], o);
-          qe[5][e] = *zp<T>(z.bc, z.pe[2] + (uint32_t)(1) * z.ps[2], o);
-        }
+      for (int t = 0; t < 3; ++t) {
+        zh[t] = *zp<T>(z.bc, z.prof[2] + (uint32_t)(3 + t) * z.pn[2], k);
+        ze[t] = *zp<T>(z.bc, z.prof[2] + (uint32_t)t * z.pn[2], kq);
       }
     }
-    // (2) prefetch plane k+1: fields, row j-1, H-side CPML memory
-    T nex[VEC], ney[VEC], nez[VEC], nhx[VEC], nhy[VEC], nhz[VEC], nexu[VEC], nezu[VEC];
-    T nqh[6][VEC];
-    const bool pf = row_ok && k + 1 <= kh;
-    if (pf) {
-      const uint32_t o = off + sz;
-      fld(nex, zp<T>(z.bi, (uint32_t)(0) * z.cs, o));
-      fld(ney, zp<T>(z.bi, (uint32_t)(1) * z.cs, o));
-      fld(nez, zp<T>(z.bi, (uint32_t)(2) * z.cs, o));
-      fld(nhx, zp<T>(z.bi, (uint32_t)(3 + 0) * z.cs, o));
-      fld(nhy, zp<T>(z.bi, (uint32_t)(3 + 1) * z.cs, o));
-      fld(nhz, zp<T>(z.bi, (uint32_t)(3 + 2) * z.cs, o));
-      if (first_row) {
-        fld(nexu, zp<T>(z.bi, (uint32_t)(0) * z.cs, o - sy));
-        fld(nezu, zp<T>(z.bi, (uint32_t)(2) * z.cs, o - sy));
-      } else {
-        fzero(nexu);
-        fzero(nezu);
-      }
-      if (pml) load_qh(k + 1, nqh);
-    } else {
-      fzero(nex);
-      fzero(ney);
-      fzero(nez);
-      fzero(nhx);
-      fzero(nhy);
-      fzero(nhz);
-      fzero(nexu);
-      fzero(nezu);
-#pragma unroll
-      for (int t = 0; t < 6; ++t) fzero(nqh[t]);
-    }
+    // (2) prefetch plane k+1 (plane kh again after the last plane: unused)
+    load_plane(min(k + 1, kh), nxt);
     // (3) share E(n) of plane k
 #pragma unroll
     for (int e = 0; e < VEC; ++e) {
-      le[0][w][x0 + e] = ex[e];
-      le[1][w][x0 + e] = ey[e];
-      le[2][w][x0 + e] = ez[e];
+      le[0][w][x0 + e] = cur.ex[e];
+      le[1][w][x0 + e] = cur.ey[e];
+      le[2][w][x0 + e] = cur.ez[e];
     }
     __syncthreads();
     // (4) H(n+1/2) on plane k
+    T hx[VEC], hy[VEC], hz[VEC];
     const bool store_h = own && k < ke;
     const int szs = (YZ && pml && row_ok) ? fslab(k, nz, g.pml[4], g.pml[5]) : -1;
     if (row_ok) {
       const bool pml_any = lane_pml_x || sys >= 0 || szs >= 0;
-      T zbhk = 1, zchk = 0, zikhk = 1;
-      if (szs >= 0) {
-        zbhk = *zp<T>(z.bc, z.prof[2] + 3u * z.pn[2], k);
-        zchk = *zp<T>(z.bc, z.prof[2] + 4u * z.pn[2], k);
-        zikhk = *zp<T>(z.bc, z.prof[2] + 5u * z.pn[2], k);
-      }
+      const T zbhk = zh[0], zchk = zh[1], zikhk = zh[2];
       const T db = (T)g.db_const;
 #pragma unroll
       for (int e = 0; e < VEC; ++e) {
         const int i = x0 + e;
-        const T exj = w > 0 ? le[0][w - 1][i] : exu[e];
-        const T ezj = w > 0 ? le[2][w - 1][i] : ezu[e];
+        const T ex = cur.ex[e], ey = cur.ey[e], ez = cur.ez[e];
+        const T exj = w > 0 ? le[0][w - 1][i] : (first_row ? cur.exu[e] : (T)0);
+        const T ezj = w > 0 ? le[2][w - 1][i] : (first_row ? cur.ezu[e] : (T)0);
         const T ezi = i > 0 ? le[2][w][i - 1] : (T)0;
         const T eyi = i > 0 ? le[1][w][i - 1] : (T)0;
-        T c1 = (ez[e] - ezj) * idy;  // Hx: dEz/dy - dEy/dz
-        T c2 = (ey[e] - eym[e]) * idz;
-        T d1 = (ex[e] - exm[e]) * idz;  // Hy: dEx/dz - dEz/dx
-        T d2 = (ez[e] - ezi) * idx;
-        T f1 = (ey[e] - eyi) * idx;  // Hz: dEy/dx - dEx/dy
-        T f2 = (ex[e] - exj) * idy;
+        T c1 = (ez - ezj) * idy;  // Hx: dEz/dy - dEy/dz
+        T c2 = (ey - eym[e]) * idz;
+        T d1 = (ex - exm[e]) * idz;  // Hy: dEx/dz - dEz/dx
+        T d2 = (ez - ezi) * idx;
+        T f1 = (ey - eyi) * idx;  // Hz: dEy/dx - dEx/dy
+        T f2 = (ex - exj) * idy;
         if (pml_any && i < nx) {
           if (sys >= 0) {
             const uint32_t o = fslab_off(g, 1, sys, i, j, k);
-            const T p1 = ybh * qh[3][e] + ych * c1, p0 = ybh * qh[2][e] + ych * f2;
-            if (store_h && i < nx - 1) *zpw<T>(z.bc, z.pho[1] + (uint32_t)(1) * z.ps[1], o) = p1;
-            if (store_h && k < nz - 1) *zpw<T>(z.bc, z.pho[1] + (uint32_t)(0) * z.ps[1], o) = p0;
+            const T p1 = ybh * cur.q[3][e] + ych * c1, p0 = ybh * cur.q[2][e] + ych * f2;
+            if (store_h && i < nx - 1) *zpw<T>(z.bc, z.pho[1] + z.ps[1], o) = p1;
+            if (store_h && k < nz - 1) *zpw<T>(z.bc, z.pho[1], o) = p0;
             c1 = c1 * yikh + p1;
             f2 = f2 * yikh + p0;
           }
           if (szs >= 0) {
             const uint32_t o = fslab_off(g, 2, szs, i, j, k);
-            const T p0 = zbhk * qh[4][e] + zchk * c2, p1 = zbhk * qh[5][e] + zchk * d1;
-            if (store_h && i < nx - 1) *zpw<T>(z.bc, z.pho[2] + (uint32_t)(0) * z.ps[2], o) = p0;
-            if (store_h && j < ny - 1) *zpw<T>(z.bc, z.pho[2] + (uint32_t)(1) * z.ps[2], o) = p1;
+            const T p0 = zbhk * cur.q[4][e] + zchk * c2, p1 = zbhk * cur.q[5][e] + zchk * d1;
+            if (store_h && i < nx - 1) *zpw<T>(z.bc, z.pho[2], o) = p0;
+            if (store_h && j < ny - 1) *zpw<T>(z.bc, z.pho[2] + z.ps[2], o) = p1;
             c2 = c2 * zikhk + p0;
             d1 = d1 * zikhk + p1;
           }
           if (sxs[e] >= 0) {
             const uint32_t o = fslab_off(g, 0, sxs[e], i, j, k);
-            const T p0 = xbh[e] * qh[0][e] + xch[e] * d2, p1 = xbh[e] * qh[1][e] + xch[e] * f1;
-            if (store_h && j < ny - 1) *zpw<T>(z.bc, z.pho[0] + (uint32_t)(0) * z.ps[0], o) = p0;
-            if (store_h && k < nz - 1) *zpw<T>(z.bc, z.pho[0] + (uint32_t)(1) * z.ps[0], o) = p1;
+            const T p0 = xbh[e] * cur.q[0][e] + xch[e] * d2;
+            const T p1 = xbh[e] * cur.q[1][e] + xch[e] * f1;
+            if (store_h && j < ny - 1) *zpw<T>(z.bc, z.pho[0], o) = p0;
+            if (store_h && k < nz - 1) *zpw<T>(z.bc, z.pho[0] + z.ps[0], o) = p1;
             d2 = d2 * xikh[e] + p0;
             f1 = f1 * xikh[e] + p1;
           }
         }
         if constexpr (CK == PM_COEFF_FULL) {
-          hx[e] = *zp<T>(z.bc, z.da[0], off + e) * hxi[e] - *zp<T>(z.bc, z.db[0], off + e) * (c1 - c2);
-          hy[e] = *zp<T>(z.bc, z.da[1], off + e) * hyi[e] - *zp<T>(z.bc, z.db[1], off + e) * (d1 - d2);
-          hz[e] = *zp<T>(z.bc, z.da[2], off + e) * hzi[e] - *zp<T>(z.bc, z.db[2], off + e) * (f1 - f2);
+          hx[e] = *zp<T>(z.bc, z.da[0], off + e) * cur.hx[e] - *zp<T>(z.bc, z.db[0], off + e) * (c1 - c2);
+          hy[e] = *zp<T>(z.bc, z.da[1], off + e) * cur.hy[e] - *zp<T>(z.bc, z.db[1], off + e) * (d1 - d2);
+          hz[e] = *zp<T>(z.bc, z.da[2], off + e) * cur.hz[e] - *zp<T>(z.bc, z.db[2], off + e) * (f1 - f2);
         } else {
-          hx[e] = hxi[e] - db * (c1 - c2);
-          hy[e] = hyi[e] - db * (d1 - d2);
-          hz[e] = hzi[e] - db * (f1 - f2);
+          hx[e] = cur.hx[e] - db * (c1 - c2);
+          hy[e] = cur.hy[e] - db * (d1 - d2);
+          hz[e] = cur.hz[e] - db * (f1 - f2);
         }
         if (i >= nx - 1) hx[e] = 0;
         if (i >= nx || j >= ny - 1) hy[e] = 0;
         if (i >= nx || k >= nz - 1) hz[e] = 0;
       }
       if (store_h) {
-        fst(zpw<T>(z.bo, (uint32_t)(3 + 0) * z.cs, off), hx);
-        fst(zpw<T>(z.bo, (uint32_t)(3 + 1) * z.cs, off), hy);
-        fst(zpw<T>(z.bo, (uint32_t)(3 + 2) * z.cs, off), hz);
+        fst(zpw<T>(z.bo, 3u * z.cs, off), hx);
+        fst(zpw<T>(z.bo, 4u * z.cs, off), hy);
+        fst(zpw<T>(z.bo, 5u * z.cs, off), hz);
       }
     } else {
       fzero(hx);
@@ -514,13 +482,9 @@ __global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, 
     if (do_e) {
       const uint32_t o = off - sz;
       const int p = kk & 1;
+      const int szz = (YZ && pml) ? fslab(kk, nz, g.pml[4], g.pml[5]) : -1;
       const bool pml_any = lane_pml_x || sys >= 0 || szz >= 0;
-      T zbek = 1, zcek = 0, zikek = 1;
-      if (szz >= 0) {
-        zbek = *zp<T>(z.bc, z.prof[2] + 0u * z.pn[2], kk);
-        zcek = *zp<T>(z.bc, z.prof[2] + 1u * z.pn[2], kk);
-        zikek = *zp<T>(z.bc, z.prof[2] + 2u * z.pn[2], kk);
-      }
+      const T zbek = ze[0], zcek = ze[1], zikek = ze[2];
       const bool ade = g.n_ade && j >= g.ade_j0 && j < g.ade_j1 && kk >= g.ade_k0 &&
                        kk < g.ade_k1;
       T ox[VEC], oy[VEC], oz[VEC];
@@ -543,24 +507,24 @@ __global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, 
           if (sys >= 0) {
             const uint32_t oe = fslab_off(g, 1, sys, i, j, kk);
             const T p1 = ybe * qe[3][e] + yce * a1, p0 = ybe * qe[2][e] + yce * c2;
-            if (exok) *zpw<T>(z.bc, z.pe[1] + (uint32_t)(1) * z.ps[1], oe) = p1;
-            if (ezok) *zpw<T>(z.bc, z.pe[1] + (uint32_t)(0) * z.ps[1], oe) = p0;
+            if (exok) *zpw<T>(z.bc, z.pe[1] + z.ps[1], oe) = p1;
+            if (ezok) *zpw<T>(z.bc, z.pe[1], oe) = p0;
             a1 = a1 * yike + p1;
             c2 = c2 * yike + p0;
           }
           if (szz >= 0) {
             const uint32_t oe = fslab_off(g, 2, szz, i, j, kk);
             const T p0 = zbek * qe[4][e] + zcek * a2, p1 = zbek * qe[5][e] + zcek * b1;
-            if (exok) *zpw<T>(z.bc, z.pe[2] + (uint32_t)(0) * z.ps[2], oe) = p0;
-            if (eyok) *zpw<T>(z.bc, z.pe[2] + (uint32_t)(1) * z.ps[2], oe) = p1;
+            if (exok) *zpw<T>(z.bc, z.pe[2], oe) = p0;
+            if (eyok) *zpw<T>(z.bc, z.pe[2] + z.ps[2], oe) = p1;
             a2 = a2 * zikek + p0;
             b1 = b1 * zikek + p1;
           }
           if (sxs[e] >= 0) {
             const uint32_t oe = fslab_off(g, 0, sxs[e], i, j, kk);
             const T p0 = xbe[e] * qe[0][e] + xce[e] * b2, p1 = xbe[e] * qe[1][e] + xce[e] * c1;
-            if (eyok) *zpw<T>(z.bc, z.pe[0] + (uint32_t)(0) * z.ps[0], oe) = p0;
-            if (ezok) *zpw<T>(z.bc, z.pe[0] + (uint32_t)(1) * z.ps[0], oe) = p1;
+            if (eyok) *zpw<T>(z.bc, z.pe[0], oe) = p0;
+            if (ezok) *zpw<T>(z.bc, z.pe[0] + z.ps[0], oe) = p1;
             b2 = b2 * xike[e] + p0;
             c1 = c1 * xike[e] + p1;
           }
@@ -594,30 +558,28 @@ __global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, 
         oy[e] = eyok ? vy : (T)0;
         oz[e] = ezok ? vz : (T)0;
       }
-      fst(zpw<T>(z.bo, (uint32_t)(0) * z.cs, o), ox);
-      fst(zpw<T>(z.bo, (uint32_t)(1) * z.cs, o), oy);
-      fst(zpw<T>(z.bo, (uint32_t)(2) * z.cs, o), oz);
+      fst(zpw<T>(z.bo, 0u, o), ox);
+      fst(zpw<T>(z.bo, z.cs, o), oy);
+      fst(zpw<T>(z.bo, 2u * z.cs, o), oz);
     }
-    // (7) rotate planes, adopt the prefetched plane
+    // (7) plane k becomes plane k-1
 #pragma unroll
     for (int e = 0; e < VEC; ++e) {
-      exm[e] = ex[e];
-      eym[e] = ey[e];
-      ezm[e] = ez[e];
+      exm[e] = cur.ex[e];
+      eym[e] = cur.ey[e];
+      ezm[e] = cur.ez[e];
       hxp[e] = hx[e];
       hyp[e] = hy[e];
       hzp[e] = hz[e];
-      ex[e] = nex[e];
-      ey[e] = ney[e];
-      ez[e] = nez[e];
-      hxi[e] = nhx[e];
-      hyi[e] = nhy[e];
-      hzi[e] = nhz[e];
-      exu[e] = nexu[e];
-      ezu[e] = nezu[e];
-#pragma unroll
-      for (int t = 0; t < 6; ++t) qh[t][e] = nqh[t][e];
     }
+  };
+
+  PlaneIn pa, pb;
+  load_plane(kb, pa);
+  for (int k = kb; k <= kh; k += 2) {
+    step(k, pa, pb);
+    if (k + 1 > kh) break;
+    step(k + 1, pb, pa);
   }
   // top plane of the grid: only Ez exists there, no z differences
   if (ke == nz && own) {
@@ -659,6 +621,26 @@ __global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, 
   }
 }
 
+// One launch for all tiles: workgroups [0, seg) take the CPML slab tiles
+// (the slower ones, dispatched first), the rest the interior box, so the
+// interior tiles fill the CUs the slab tiles leave idle at the end.  seg is
+// a multiple of 8, so a workgroup's XCD (blockIdx % 8) is the same in its
+// segment's numbering.
+template <typename T, int CK, int VEC, int W, int TY>
+__global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, const KFz z,
+                                                                 int kchunk, int ntile_y,
+                                                                 int nchunk, int n_out,
+                                                                 int n_in, int seg, int order,
+                                                                 TileBox bx) {
+  __shared__ FusedLds<T, VEC, W, TY> lds;
+  const int b = blockIdx.x;
+  if (b < seg)
+    fused_tile<T, CK, VEC, W, TY, true>(g, z, kchunk, ntile_y, nchunk, n_out, order, bx, b, lds);
+  else
+    fused_tile<T, CK, VEC, W, TY, false>(g, z, kchunk, ntile_y, nchunk, n_in, order, bx,
+                                         b - seg, lds);
+}
+
 template <typename T, int VEC, int W, int TY>
 constexpr bool fused_fits() {
   return 64 * W * (TY + 1) <= 1024 && (size_t)9 * (TY + 1) * 64 * VEC * W * sizeof(T) <= 65536;
@@ -685,13 +667,10 @@ int launch_fused_one(const KGrid& g, const KFz& z, int kchunk, int order, hipStr
     }
     const int n_in = (bx.ty1 - bx.ty0) * (bx.kc1 - bx.kc0);
     const int n_out = ntile_y * nchunk - n_in;
-    const dim3 block(64 * W * (TY + 1));
-    if (n_in > 0)
-      hipLaunchKernelGGL((k_yee_fused<T, CK, VEC, W, TY, false>), dim3(((n_in + 7) / 8) * 8),
-                         block, 0, s, g, z, kchunk, ntile_y, nchunk, n_in, order, bx);
-    if (n_out > 0)
-      hipLaunchKernelGGL((k_yee_fused<T, CK, VEC, W, TY, true>), dim3(((n_out + 7) / 8) * 8),
-                         block, 0, s, g, z, kchunk, ntile_y, nchunk, n_out, order, bx);
+    const int seg = ((n_out + 7) / 8) * 8;
+    const int nwg = seg + ((n_in + 7) / 8) * 8;
+    hipLaunchKernelGGL((k_yee_fused<T, CK, VEC, W, TY>), dim3(nwg), dim3(64 * W * (TY + 1)), 0,
+                       s, g, z, kchunk, ntile_y, nchunk, n_out, n_in, seg, order, bx);
     return 0;
   } else {
     return -1;
@@ -792,7 +771,14 @@ inline bool make_kfz_impl(const KGrid& g, const FusedBufs& fb, int esz, KFz* out
   for (int a = 0; a < 3; ++a) {
     if (!off(g.ca[a], &z.ca[a]) || !off(g.da[a], &z.da[a]) || !off(g.db[a], &z.db[a]))
       PM_KFZ_FAIL("Ca/Da/Db beyond 4 GiB");
-    if (g.pml[2 * a] + g.pml[2 * a + 1] == 0) continue;
+    if (g.pml[2 * a] + g.pml[2 * a + 1] == 0) {
+      // no CPML on this axis: the kernel's unconditional CPML-memory loads
+      // (slab position 0, values unused) read the Cb arrays instead, which
+      // are larger than any position they can form
+      z.pe[a] = z.phi[a] = z.pho[a] = z.prof[a] = z.cb;
+      z.ps[a] = z.pn[a] = 0;
+      continue;
+    }
     // CPML memory pairs back to back, profiles back to back
     const uint64_t S = (uint64_t)(g.pml[2 * a] + g.pml[2 * a + 1]);
     const uint64_t ps = S * (uint64_t)g.nx * g.ny * g.nz / (uint64_t)n[a] * (uint64_t)esz;
