@@ -262,7 +262,10 @@ def main():
     print(f"[bench] fused step launch: {tuned}", file=sys.stderr, flush=True)
     sim.run_steps(args.warmup)
     torch.cuda.synchronize()
-    eng.ctx.set_timing(True)
+    # HIP events around the kernels of every 10th step (every step of a
+    # short run): enough launches for a stable average, few enough that the
+    # events do not slow the run
+    eng.ctx.set_timing(True, stride=max(1, min(10, args.steps // 20)))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     sim.run_steps(args.steps)

@@ -231,7 +231,7 @@ int pm_run(pm_ctx* ctx, int64_t step0, int32_t nsteps, uint32_t flags);
  * [step_first, step_first + rows), into every DFT group (partial block). */
 int pm_dft_flush(pm_ctx* ctx, int64_t step_first, int32_t rows);
 
-/* Launch shape of the fused step: rows per workgroup (ty in {2, 4}), z planes
+/* Launch shape of the fused step: rows per workgroup (ty in {2, 3, 4}), z planes
  * per workgroup (kchunk >= 1), cells per lane (vec in {1, 2, 4}, 0 = auto)
  * and tile order (0: y-tiles fastest, 1: z-chunks fastest, each XCD taking a
  * contiguous range; 2: y-tiles fastest in hardware round-robin); fuse = 0 forces
@@ -248,7 +248,9 @@ int pm_sync(pm_ctx* ctx);
  * events on the context stream around every launch of kernel class `which`
  * (0 = H update, 1 = E update, 2 = whole step, 3 = fused H+E step,
  * 4 = spectral flush of one sample block); enabled (and reset) by
- * pm_set_timing. */
+ * pm_set_timing.  enable = 0 turns timing off; enable = s >= 1 times the
+ * kernels of every s-th step of pm_run (steps whose index is a multiple of
+ * s), so a long run can be timed with few events. */
 int pm_set_timing(pm_ctx* ctx, int enable);
 int pm_get_timing(pm_ctx* ctx, int which, double* avg_ms, int64_t* count);
 

@@ -255,8 +255,10 @@ class Context:
     def sync(self):
         self._check(self.lib.pm_sync(self.h), "pm_sync")
 
-    def set_timing(self, on: bool):
-        self._check(self.lib.pm_set_timing(self.h, 1 if on else 0), "pm_set_timing")
+    def set_timing(self, on, stride: int = 1):
+        """Time the kernels of every ``stride``-th step (off when ``on`` is false)."""
+        self._check(self.lib.pm_set_timing(self.h, max(1, int(stride)) if on else 0),
+                    "pm_set_timing")
 
     def timing(self, which: int):
         ms = f64()

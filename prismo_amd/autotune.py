@@ -19,8 +19,8 @@ _CACHE: dict = {}
 
 CANDIDATES = {
     "vec": (1, 2),
-    "ty": (2, 4),
-    "kchunk": (4, 8, 16),
+    "ty": (2, 3, 4),
+    "kchunk": (8, 16, 32),
     "order": (0, 2),
 }
 

@@ -30,6 +30,8 @@ struct pm_ctx {
   std::vector<KAde> ade_host;  // staging of the ADE table upload
   // timing
   bool timing = false;
+  int timing_stride = 1;  // time the kernels of every timing_stride-th step
+  bool span_on = false;   // timing, and the current step is one of those
   std::vector<hipEvent_t> pool;
   size_t pool_used = 0;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> spans[5];
@@ -52,10 +54,13 @@ int hip_check(pm_ctx* c, hipError_t e, const char* what) {
   return fail(c, (int)e, "%s: %s", what, hipGetErrorString(e));
 }
 
+// Timing events skip the system-scope fence on record: they only time the
+// stream's own work (elapsed times are read after a stream synchronize), and
+// the fence's cache writeback would otherwise add idle time between kernels.
 hipEvent_t take_event(pm_ctx* c) {
   if (c->pool_used == c->pool.size()) {
     hipEvent_t e;
-    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
     c->pool.push_back(e);
   }
   return c->pool[c->pool_used++];
@@ -66,13 +71,13 @@ struct Span {
   int which;
   hipEvent_t a = nullptr;
   Span(pm_ctx* ctx, int w) : c(ctx), which(w) {
-    if (c->timing) {
+    if (c->span_on) {
       a = take_event(c);
       if (a) (void)hipEventRecord(a, c->stream);
     }
   }
   ~Span() {
-    if (c->timing && a) {
+    if (a) {
       hipEvent_t b = take_event(c);
       if (b) {
         (void)hipEventRecord(b, c->stream);
@@ -569,6 +574,7 @@ int pm_run(pm_ctx* c, int64_t step0, int32_t nsteps, uint32_t flags) {
   const bool fused = c->fused_ok && c->fuse && fused_shape(c, &fvec, &fw, &fty);
   for (int32_t s = 0; s < nsteps; ++s) {
     const int64_t n = step0 + s;
+    c->span_on = c->timing && n % c->timing_stride == 0;
     Span whole(c, 2);
     int r;
     // fused runs keep an even number of ping-pong steps: an odd run does its
@@ -606,6 +612,7 @@ int pm_run(pm_ctx* c, int64_t step0, int32_t nsteps, uint32_t flags) {
       }
     }
   }
+  c->span_on = c->timing;
   return 0;
 }
 
@@ -621,7 +628,7 @@ int pm_dft_flush(pm_ctx* c, int64_t /*step_first*/, int32_t /*rows*/) {
 
 int pm_set_tuning(pm_ctx* c, int fuse, int ty, int kchunk, int vec, int order) {
   if (!c) return PM_EINVAL;
-  if (ty != 2 && ty != 4) return fail(c, PM_EINVAL, "ty must be 2 or 4");
+  if (ty < 2 || ty > 4) return fail(c, PM_EINVAL, "ty must be 2, 3 or 4");
   if (kchunk < 1) return fail(c, PM_EINVAL, "kchunk must be >= 1");
   if (vec != 0 && vec != 1 && vec != 2 && vec != 4)
     return fail(c, PM_EINVAL, "vec must be 0 (auto), 1, 2 or 4");
@@ -648,7 +655,10 @@ int pm_sync(pm_ctx* c) {
 
 int pm_set_timing(pm_ctx* c, int enable) {
   if (!c) return PM_EINVAL;
+  if (enable < 0) return fail(c, PM_EINVAL, "negative timing stride");
   c->timing = enable != 0;
+  c->timing_stride = enable > 0 ? enable : 1;
+  c->span_on = c->timing;
   c->pool_used = 0;
   for (auto& v : c->spans) v.clear();
   return 0;

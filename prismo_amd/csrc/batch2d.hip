@@ -332,7 +332,8 @@ struct BSpan {
   int which;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   BSpan(pm_batch* bb, int w) : b(bb), which(w) {
-    if (b->timing && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess)
+    if (b->timing && hipEventCreateWithFlags(&e0, hipEventDisableSystemFence) == hipSuccess &&
+        hipEventCreateWithFlags(&e1, hipEventDisableSystemFence) == hipSuccess)
       (void)hipEventRecord(e0, b->stream);
   }
   ~BSpan() {

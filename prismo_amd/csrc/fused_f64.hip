@@ -14,7 +14,7 @@ int launch_yee_fused_f64(const KGrid& g, int ck, const FusedBufs& fb, int vec, i
 
 bool fused_supported(int precision, int vec, int w, int ty) {
   const size_t t = precision == PM_FP64 ? 8 : 4;
-  if (ty != 2 && ty != 4) return false;
+  if (ty < 2 || ty > 4) return false;
   const bool pair = (vec == 1 && (w == 2 || w == 4)) || (vec == 2 && (w == 1 || w == 2 || w == 4)) ||
                     (vec == 4 && (w == 1 || w == 2));
   if (!pair) return false;

@@ -299,6 +299,9 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
   const int jr = min(j, ny - 1);
   const uint32_t rowoff = (uint32_t)jr * sy + (uint32_t)x0;
   const uint32_t upoff = (uint32_t)(first_row ? j - 1 : jr) * sy + (uint32_t)x0;
+  // the halo row takes no E update: its Cb loads read the row above, which
+  // the tile's last own row fetches at the same time (a cache hit)
+  const uint32_t cboff = (uint32_t)(w < TY ? jr : min(j - 1, ny - 1)) * sy + (uint32_t)x0;
   int sxc[VEC];
 #pragma unroll
   for (int e = 0; e < VEC; ++e) sxc[e] = max(sxs[e], 0);
@@ -373,7 +376,7 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
     //     plane's prefetch: vector memory counters retire in order, so the
     //     wait for these does not wait for the prefetch
     CbRaw<T, CK> cbx[VEC], cby[VEC], cbz[VEC];
-    const uint32_t oq = rowoff + (uint32_t)kq * sz;
+    const uint32_t oq = cboff + (uint32_t)kq * sz;
     fcb<T, CK, VEC>(z, 0, oq, cbx);
     fcb<T, CK, VEC>(z, 1, oq, cby);
     fcb<T, CK, VEC>(z, 2, oq, cbz);
@@ -681,6 +684,7 @@ template <typename T, int CK, int VEC, int W>
 int launch_fused_w(const KGrid& g, const KFz& z, int ty, int kchunk, int order,
                    hipStream_t s) {
   if (ty == 2) return launch_fused_one<T, CK, VEC, W, 2>(g, z, kchunk, order, s);
+  if (ty == 3) return launch_fused_one<T, CK, VEC, W, 3>(g, z, kchunk, order, s);
   if (ty == 4) return launch_fused_one<T, CK, VEC, W, 4>(g, z, kchunk, order, s);
   return -1;
 }
