@@ -234,7 +234,8 @@ int pm_dft_flush(pm_ctx* ctx, int64_t step_first, int32_t rows);
 /* Launch shape of the fused step: rows per workgroup (ty in {2, 3, 4}), z planes
  * per workgroup (kchunk >= 1), cells per lane (vec in {1, 2, 4}, 0 = auto)
  * and tile order (0: y-tiles fastest, 1: z-chunks fastest, each XCD taking a
- * contiguous range; 2: y-tiles fastest in hardware round-robin); fuse = 0 forces
+ * contiguous range; 2: y-tiles fastest in hardware round-robin; 3: as 2 in
+ * groups of 64 tiles of which each XCD takes 8 neighbours); fuse = 0 forces
  * the unfused kernels.  Defaults: 1, 4, 8, 0, 2.  Used by the autotuner. */
 int pm_set_tuning(pm_ctx* ctx, int fuse, int ty, int kchunk, int vec, int order);
 

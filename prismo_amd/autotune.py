@@ -20,8 +20,8 @@ _CACHE: dict = {}
 CANDIDATES = {
     "vec": (1, 2),
     "ty": (2, 3, 4),
-    "kchunk": (8, 16, 32),
-    "order": (0, 2),
+    "kchunk": (8, 12, 16, 24),
+    "order": (2, 3),
 }
 
 

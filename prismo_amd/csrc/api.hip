@@ -632,7 +632,7 @@ int pm_set_tuning(pm_ctx* c, int fuse, int ty, int kchunk, int vec, int order) {
   if (kchunk < 1) return fail(c, PM_EINVAL, "kchunk must be >= 1");
   if (vec != 0 && vec != 1 && vec != 2 && vec != 4)
     return fail(c, PM_EINVAL, "vec must be 0 (auto), 1, 2 or 4");
-  if (order < 0 || order > 2) return fail(c, PM_EINVAL, "order must be 0, 1 or 2");
+  if (order < 0 || order > 3) return fail(c, PM_EINVAL, "order must be 0, 1, 2 or 3");
   c->fuse = fuse != 0;
   c->tune_ty = ty;
   c->tune_kchunk = kchunk;

@@ -207,8 +207,18 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
   auto& lh = lds.lh;
   const int per = (nblocks + 7) / 8;
   // orders 0/1: XCD b % 8 runs a contiguous range of tiles (halo reuse in its
-  // L2); order 2: hardware round-robin (slab tiles spread over all XCDs)
-  const int L = order == 2 ? b : (b % 8) * per + b / 8;
+  // L2); order 2: hardware round-robin (slab tiles spread over all XCDs);
+  // order 3: the tile list in groups of 64, XCD x taking tiles 8x .. 8x+7 of
+  // each group, so the XCDs advance through the grid together while each
+  // runs runs of 8 neighbouring tiles (a remainder under 64 goes round-robin)
+  int L;
+  if (order == 2) {
+    L = b;
+  } else if (order == 3) {
+    L = b < (nblocks & ~63) ? (b & ~63) + (b & 7) * 8 + ((b >> 3) & 7) : b;
+  } else {
+    L = (b % 8) * per + b / 8;
+  }
   if (L >= nblocks) return;
   int ty, kc;
   if constexpr (!YZ) {
@@ -345,7 +355,9 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
     fld(P.ey, zp<T>(z.bi, z.cs, o));
     fld(P.ez, zp<T>(z.bi, 2u * z.cs, o));
     fld(P.hx, zp<T>(z.bi, 3u * z.cs, o));
-    fld(P.hy, zp<T>(z.bi, 4u * z.cs, o));
+    // the halo row's Hy feeds nothing (E of the last own row takes Hx and Hz
+    // from it): read the row above instead, a cache hit
+    fld(P.hy, zp<T>(z.bi, 4u * z.cs, w < TY ? o : cboff + (uint32_t)kq * sz));
     fld(P.hz, zp<T>(z.bi, 5u * z.cs, o));
     const uint32_t u = upoff + (uint32_t)kq * sz;
     fld(P.exu, zp<T>(z.bi, 0u, u));

@@ -82,7 +82,8 @@ def test_fused_tilings_agree(mi355x):
     stream, so the compiler may fuse different multiply-adds (<= 1e-13)."""
     ref = None
     for ty, kc, vec, order in ((4, 16, 1, 0), (2, 1, 1, 0), (4, 5, 1, 2), (2, 3, 2, 1),
-                               (4, 7, 2, 0), (2, 16, 2, 2), (3, 16, 1, 2), (3, 4, 2, 0)):
+                               (4, 7, 2, 0), (2, 16, 2, 2), (3, 16, 1, 2), (3, 4, 2, 0),
+                               (3, 2, 1, 3)):
         sim, _ = _sim(128, 22, 19, t=4)
         eng = sim.solver.updater.engine_for(sim.fields)
         eng.ctx.set_tuning(True, ty, kc, vec, order)
@@ -99,7 +100,8 @@ def test_fused_tilings_agree_wide_rows(mi355x):
     largest workgroup (1024 lanes); it, the 2-row shape and two cells per
     lane agree to rounding (FP32 fields, FP16 Cb, ragged y and z)."""
     ref = None
-    for ty, kc, vec, order in ((2, 16, 1, 2), (3, 16, 1, 2), (3, 5, 1, 0), (2, 8, 2, 1)):
+    for ty, kc, vec, order in ((2, 16, 1, 2), (3, 16, 1, 2), (3, 5, 1, 0), (2, 8, 2, 1),
+                               (3, 2, 1, 3)):
         sim, _ = _sim(256, 23, 21, t=4, dtype=np.float32, storage="fp16")
         eng = sim.solver.updater.engine_for(sim.fields)
         eng.ctx.set_tuning(True, ty, kc, vec, order)
