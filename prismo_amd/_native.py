@@ -129,6 +129,22 @@ class pm_batch_plan(C.Structure):
     ]
 
 
+class pm_shape(C.Structure):
+    _fields_ = [
+        ("kind", i32), ("material", i32), ("axis", i32), ("n_vert", i32),
+        ("vert0", i64),
+        ("center", f64 * 3), ("half", f64 * 3),
+        ("radius", f64), ("half_height", f64), ("zmin", f64), ("zmax", f64),
+    ]
+
+
+CB_F64, CB_F32, CB_F16 = 0, 1, 2
+GEOM_SIGNATURES = {
+    "pm_raster": ([vp, i32, i32, i32, vp, vp, vp, vp, i32, vp, i64, vp], C.c_int),
+    "pm_cb_from_index": ([vp, i32, i32, i32, vp, vp, i32, i32, i32, f64, i64, i64, vp, vp],
+                         C.c_int),
+}
+
 # Exported symbols and their signatures (the full set declared in the headers).
 MLP_LINEAR, MLP_GELU, MLP_DGELU, MLP_PLAIN = 0, 1, 2, 3
 fp = C.POINTER(C.c_float)
@@ -192,7 +208,8 @@ def load(path: os.PathLike | None = None):
             "(the mi355x time step has no CPU fallback)"
         )
     lib = C.CDLL(str(p))
-    for name, (args, res) in {**SIGNATURES, **BATCH_SIGNATURES, **MLP_SIGNATURES}.items():
+    for name, (args, res) in {**SIGNATURES, **BATCH_SIGNATURES, **MLP_SIGNATURES,
+                              **GEOM_SIGNATURES}.items():
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = res

@@ -38,6 +38,7 @@ UNITS = {
     "dft_mfma.hip": [],
     "batch2d.hip": [],
     "mlp.hip": [],
+    "geom.hip": ["-ffp-contract=off"],
     "fused_f32.hip": [],
     "fused_f64.hip": [],
 }

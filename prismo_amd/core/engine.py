@@ -16,7 +16,7 @@ from typing import Optional
 import numpy as np
 
 from .. import _native as N
-from .coefficients import component_average, is_lossless_nonmagnetic
+from .coefficients import cell_coefficients, component_average, is_lossless_nonmagnetic
 from .grid import COMPONENTS
 
 EPS0 = 8.854187817e-12
@@ -186,6 +186,8 @@ class StepEngine:
             self.need_h = ["Hx", "Hy", "Hz"]
 
     def _fill_coefficients(self, coeffs, storage):
+        if len(coeffs) == 3 and isinstance(coeffs[0], str) and coeffs[0] == "device":
+            return self._fill_coefficients_device(coeffs[1], coeffs[2], storage)
         torch = _torch()
         Ca, Cb, Da, Db, mu_rel, sig_e, sig_m = coeffs
         p = self.plan
@@ -211,6 +213,7 @@ class StepEngine:
                 raise ValueError("uniform Db expected for lossless non-magnetic media")
             p.db_const = d0
         self.cb_host = {}
+        self.cb_dev = {}
         nx, ny, nz = self.grid.dimensions
         cb_dt = torch.float16 if kind == N.COEFF_CB_F16 else self.tdtype
         # the three Cb arrays back to back (the fused kernel's layout)
@@ -223,10 +226,11 @@ class StepEngine:
             if kind == N.COEFF_CB_F16:
                 q = (cb / self.cb_scale).astype(np.float16)
                 self.cb_host[c] = q.astype(np.float64) * self.cb_scale
-                p.cb[a] = self._padded(q, c, torch.float16, dst).data_ptr()
+                self.cb_dev[c] = self._padded(q, c, torch.float16, dst)
             else:
                 self.cb_host[c] = cb
-                p.cb[a] = self._padded(cb, c, self.tdtype, dst).data_ptr()
+                self.cb_dev[c] = self._padded(cb, c, self.tdtype, dst)
+            p.cb[a] = self.cb_dev[c].data_ptr()
             if kind == N.COEFF_FULL:
                 p.ca[a] = self._padded(component_average(sl(Ca), c, self.is_2d), c,
                                        self.tdtype).data_ptr()
@@ -238,6 +242,71 @@ class StepEngine:
                                        self.tdtype).data_ptr()
                 p.db[a] = self._padded(component_average(sl(Db), c, self.is_2d), c,
                                        self.tdtype).data_ptr()
+
+    def _fill_coefficients_device(self, shapes, background_eps, storage):
+        """Cb from shapes painted on the device (include/pmgeom.h): a
+        material index per node, then per E component the average of the
+        materials' node Cb, straight into the padded arrays.  Same bits as
+        the host path on the equivalent node arrays."""
+        from ..geometry import raster
+        from ..geometry.shapes import Material
+
+        torch = _torch()
+        if self.scheme != "yee":
+            raise NotImplementedError("device-painted shapes need scheme='yee'")
+        if storage == "fp16":
+            if self.precision != N.FP32:
+                raise ValueError("FP16 coefficients need FP32 fields")
+            kind = N.COEFF_CB_F16
+        elif storage in ("auto", "native"):
+            kind = N.COEFF_CB
+        else:
+            raise ValueError("device-painted shapes take coefficient_storage 'auto' or 'fp16'")
+        p = self.plan
+        self.coeff_kind = kind
+        p.coeff_kind = kind
+        self.cb_scale = self.dt / EPS0
+        p.cb_scale = self.cb_scale
+        mats = raster.material_table(shapes, Material("background", background_eps))
+        eps = np.array([m.epsilon_r for m in mats], dtype=np.float64)
+        one, zero = np.ones_like(eps), np.zeros_like(eps)
+        _, cb_lut, _, db = cell_coefficients(eps, one, zero, zero, self.dt)
+        p.db_const = float(db[0])
+        g = self.grid
+        nx, ny, nz = g.dimensions
+        x = g.origin[0] + np.arange(nx) * g.dx
+        y = g.origin[1] + np.arange(ny) * g.dy
+        z = np.zeros(1) if self.is_2d else g.origin[2] + np.arange(nz) * g.dz
+        idx, _ = raster.paint(shapes, x, y, z, self.device, mats)
+        self.material_index = idx
+        self.materials = mats
+        self.cb_host = {}
+        self.cb_dev = {}
+        cb_dt = torch.float16 if kind == N.COEFF_CB_F16 else self.tdtype
+        store = {N.COEFF_CB_F16: N.CB_F16}.get(kind, N.CB_F64 if self.precision == N.FP64
+                                              else N.CB_F32)
+        cb_all = None if self.is_2d else self._carve((3, nz, ny, self.sy), cb_dt)
+        for a, c in enumerate(("Ex", "Ey", "Ez")):
+            if c not in self.need_e:
+                continue
+            buf = self._alloc((nz, ny, self.sy), cb_dt) if cb_all is None else cb_all[a]
+            raster.cb_from_index(idx, cb_lut, a, store, self.cb_scale, buf, self.sy, self.sz)
+            self.cb_dev[c] = buf
+            p.cb[a] = buf.data_ptr()
+
+    def coefficient_at(self, comp: str, box) -> tuple:
+        """(Cb, Db) in FP64 at the cells of ``box`` ((i0, i1), (j0, j1),
+        (k0, k1)) of E component ``comp`` and of the matching H points, read
+        back from the device arrays (lossless media: Db is one constant)."""
+        if self.coeff_kind == N.COEFF_FULL:
+            raise NotImplementedError("coefficients of lossy or magnetic media")
+        (i0, i1), (j0, j1), (k0, k1) = box
+        t = self.cb_dev[comp][k0:k1, j0:j1, i0:i1].permute(2, 1, 0).double().cpu().numpy()
+        if self.coeff_kind == N.COEFF_CB_F16:
+            t = t * self.cb_scale
+        if self.is_2d:
+            t = t[:, :, 0]
+        return t, float(self.plan.db_const)
 
     def _fill_cpml(self, cpml):
         p = self.plan

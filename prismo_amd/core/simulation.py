@@ -55,12 +55,28 @@ class Simulation:
 
     # -- set-up -------------------------------------------------------------------
     def add_source(self, source) -> None:
+        if hasattr(source, "bind_simulation"):
+            source.bind_simulation(self)
         source.initialize(self.grid)
         self.sources.append(source)
 
     def add_monitor(self, monitor) -> None:
         monitor.initialize(self.grid)
         self.monitors.append(monitor)
+
+    def add_structure(self, shape) -> None:
+        """Paint ``shape`` (prismo_amd.geometry) over the structures added
+        before it; the grid is rasterised and the coefficients built on the
+        device when the simulation first steps.  Lossless, non-magnetic
+        materials; the background is ``set_background`` (vacuum by default)."""
+        self._structures = getattr(self, "_structures", []) + [shape]
+        self.solver.updater.set_structures(self._structures, getattr(self, "_background", 1.0))
+
+    def set_background(self, eps_rel: float) -> None:
+        """Relative permittivity of the nodes no structure covers."""
+        self._background = float(getattr(eps_rel, "epsilon_r", eps_rel))
+        if getattr(self, "_structures", None):
+            self.solver.updater.set_structures(self._structures, self._background)
 
     def add_dispersive_box(self, material, box_min, box_max) -> list:
         """Make every E sample point inside [box_min, box_max] dispersive

@@ -59,6 +59,22 @@ class MaxwellUpdater:
         self._initialize_materials(material_arrays)
         self._compute_update_coefficients()
         self._engines: dict = {}
+        self.structures: list = []
+        self.background_eps = 1.0
+        self._has_material_arrays = bool(material_arrays)
+
+    def set_structures(self, shapes, background_eps: float = 1.0) -> None:
+        """Shapes painted on the device (later shapes on top of earlier ones,
+        ``background_eps`` elsewhere) instead of the host material arrays;
+        lossless, non-magnetic materials only."""
+        if self._has_material_arrays:
+            raise ValueError("give the structure either as material_arrays or as shapes")
+        for sh in shapes:
+            if getattr(sh.material, "mu_r", 1.0) != 1.0:
+                raise NotImplementedError("device-painted shapes take non-magnetic materials")
+        self.structures = list(shapes)
+        self.background_eps = float(background_eps)
+        self.invalidate()
 
     # -- set-up (host FP64) ---------------------------------------------------
     def _initialize_materials(self, material_arrays) -> None:
@@ -123,9 +139,13 @@ class MaxwellUpdater:
         key = id(fields)
         eng = self._engines.get(key)
         if eng is None or eng.fields is not fields:
+            if self.structures:
+                coeffs = ("device", self.structures, self.background_eps)
+            else:
+                coeffs = (self.Ca, self.Cb, self.Da, self.Db, self.mu_rel, self.sigma_e,
+                          self.sigma_m)
             eng = StepEngine(
-                self.grid, self.dt, fields,
-                (self.Ca, self.Cb, self.Da, self.Db, self.mu_rel, self.sigma_e, self.sigma_m),
+                self.grid, self.dt, fields, coeffs,
                 scheme=self.scheme, cpml=self.cpml, ade=self.ade,
                 coefficient_storage=self.coefficient_storage,
             )

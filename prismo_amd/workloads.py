@@ -182,3 +182,81 @@ def coupler_analyze(sim, params=None) -> dict:
     return {"s21": s21, "s31": s31, "p_in": p_in,
             "t21": s21 * np.exp(1j * (phase(ports["thru"]) - ph_in)),
             "t31": s31 * np.exp(1j * (phase(ports["cross"]) - ph_in))}
+
+
+# ------------------------------------------------------- dispersive waveguide
+WG_FREQS = np.linspace(180e12, 210e12, 64)
+WG_STEPS = 6000
+WG_DIMS = (480, 240, 160)
+WG_PML = 12
+
+
+def waveguide_3d(dtype=np.float32, coefficient_storage="auto", steps=WG_STEPS, backend=None):
+    """Dispersive silicon waveguide, 480 x 240 x 160 total cells at 20 nm
+    (12-cell CPML inside): a 450 x 220 nm Lorentz-silicon core along x
+    (eps_inf 7.98, delta-eps 3.69, w0 1.1e16, gamma 1e13) on an SiO2
+    substrate (eps 2.085), air above, and a 200 x 200 x 50-cell Drude pad
+    (wp 1.37e16, gamma 1.07e14, eps_inf 1) 300 nm above the core over x in
+    [4, 8] um.  The structure is painted on the device; the fundamental TE
+    mode (solved at 193 THz with the core's Lorentz eps) is launched along +x
+    from x = 1 um with a 193 THz Gaussian pulse (10 fs); the y-z plane at
+    x = 8.6 um records Ey, Ez, Hy, Hz at 64 frequencies (180-210 THz).
+    Returns (sim, monitor, mode, steps)."""
+    import prismo_amd as pa
+
+    nx, ny, nz = WG_DIMS
+    p = WG_PML
+    size = tuple((n - 2 * p) * D * SHRINK for n in WG_DIMS)
+    sim = pa.Simulation(size=size, resolution=1 / D, pml_layers=p, courant_factor=0.99,
+                        scheme="yee", dtype=dtype, backend=backend,
+                        pml=pa.PMLParams(thickness=p), coefficient_storage=coefficient_storage)
+    yc = size[1] / 2
+    z_sub = 1.2e-6                     # substrate top
+    core = ((0.0, yc - 225e-9, z_sub), (size[0], yc + 225e-9, z_sub + 220e-9))
+    pad = ((4.0e-6, yc - 2.0e-6, z_sub + 520e-9), (8.0e-6, yc + 2.0e-6, z_sub + 1.52e-6))
+    si = pa.LorentzMaterial(7.98, [pa.LorentzPole(omega_0=1.1e16, delta_epsilon=3.69,
+                                                  gamma=1e13)], name="Si")
+    au = pa.DrudeMaterial(1.0, omega_p=1.37e16, gamma=1.07e14, name="pad")
+    box = lambda lo, hi, m: pa.Box(m, tuple((a + b) / 2 for a, b in zip(lo, hi)),
+                                   tuple(b - a for a, b in zip(lo, hi)))
+    mats = {"SiO2": pa.Material("SiO2", 2.085), "Si": pa.Material("Si", si.epsilon_inf)}
+    big = 4 * max(size)
+    shapes = [box((-big, -big, -big), (big, big, z_sub), mats["SiO2"]),
+              box(core[0], core[1], mats["Si"])]
+    for s in shapes:
+        sim.add_structure(s)
+    sim.add_dispersive_box(si, core[0], core[1])
+    sim.add_dispersive_box(au, pad[0], pad[1])
+
+    f0 = 193e12
+    g = sim.grid
+    y = g.origin[1] + np.arange(g.dimensions[1]) * g.dy
+    z = g.origin[2] + np.arange(g.dimensions[2]) * g.dz
+    Y, Z = np.meshgrid(y, z, indexing="ij")
+    pts = np.column_stack([np.full(Y.size, 1e-6), Y.ravel(), Z.ravel()])
+    eps = np.ones(Y.size)
+    eps[shapes[0].contains(pts)] = 2.085
+    eps[shapes[1].contains(pts)] = float(np.real(si.permittivity(2 * np.pi * f0)))
+    mode = pa.ModeSolver(299792458.0 / f0, y, z, eps.reshape(Y.shape)).solve(1, "TE")[0]
+    wf = pa.sources.waveform.GaussianPulse(f0, 10e-15)
+    sim.add_source(pa.ModeSource(center=(1e-6, yc, size[2] / 2), size=(0.0, size[1], size[2]),
+                                 mode=mode, direction="+x", waveform=wf))
+    mon = pa.FieldMonitor(center=(8.6e-6, yc, size[2] / 2), size=(0.0, size[1] + 2 * p * D,
+                                                                    size[2] + 2 * p * D),
+                          components=["Ey", "Ez", "Hy", "Hz"], time_domain=False,
+                          frequencies=list(WG_FREQS))
+    sim.add_monitor(mon)
+    return sim, mon, mode, steps
+
+
+def waveguide_flux(mon) -> np.ndarray:
+    """Power through the monitor plane along +x per frequency:
+    0.5 Re sum(Ey Hz* - Ez Hy*) over the common sample window."""
+    out = []
+    for f in mon.frequencies:
+        d = {c: np.squeeze(mon.get_frequency_data(c, f)) for c in ("Ey", "Ez", "Hy", "Hz")}
+        s0 = min(a.shape[0] for a in d.values())
+        s1 = min(a.shape[1] for a in d.values())
+        d = {c: a[:s0, :s1] for c, a in d.items()}
+        out.append(0.5 * np.real(d["Ey"] * np.conj(d["Hz"]) - d["Ez"] * np.conj(d["Hy"])).sum())
+    return np.array(out)

@@ -25,7 +25,7 @@ def declared_functions():
 
 def test_header_declares_what_binding_binds():
     assert set(declared_functions()) == (set(N.SIGNATURES) | set(N.BATCH_SIGNATURES)
-                                         | set(N.MLP_SIGNATURES))
+                                         | set(N.MLP_SIGNATURES) | set(N.GEOM_SIGNATURES))
 
 
 def test_library_loads_and_exports_all_symbols():
@@ -47,6 +47,7 @@ PROBE = r"""
 #include <stdio.h>
 #include "pmfdtd.h"
 #include "pmbatch.h"
+#include "pmgeom.h"
 #define P(T, f) printf(#T "." #f " %zu\n", offsetof(T, f));
 int main(void) {
   %FIELDS%
@@ -56,7 +57,7 @@ int main(void) {
 
 
 STRUCTS = (N.pm_plan, N.pm_source, N.pm_sample_box, N.pm_dft_group, N.pm_ade_box,
-           N.pm_batch_line, N.pm_batch_design, N.pm_batch_plan)
+           N.pm_batch_line, N.pm_batch_design, N.pm_batch_plan, N.pm_shape)
 
 
 def test_struct_layout_matches_header(tmp_path):
