@@ -28,6 +28,15 @@ FP64 runs of an evenly spaced sample of the same designs through
 Simulation.run; ``max_abs_s21_err`` is the largest |S21| difference
 between the two over that sample and all 64 frequencies.
 
+``strip_2d``: the 2-D TM strip (540 x 290 cells, FP64, 3000 steps) in the
+reference-arithmetic and textbook schemes on the GPU, beside the O0 NumPy
+restatement of the reference's step on one host core (a bounded number of
+steps); small grids are launch-bound, so it reports the microseconds per
+step.  ``waveguide_3d``: the dispersive waveguide (480 x 240 x 160 cells,
+Lorentz core, Drude pad, mode source, 64-frequency DFT plane, 6000 steps) in
+FP32 and FP64 with the relative L2 difference of the plane's spectra.
+``--no-strip`` / ``--no-waveguide`` / ``--no-sweep`` skip those parts.
+
 ``--surrogate`` runs only the surrogate-screened coupler search (4096
 FDTD training spectra, a 4-member BF16 MLP ensemble, 65536 screened
 designs, FDTD judge of the top 64 and the uncertain ones) and prints its
@@ -69,21 +78,19 @@ def measured_traffic(workload_key: str, launch: dict):
     it covers this workload and launch shape (else None)."""
     if not os.path.exists(TRAFFIC_FILE):
         return None
-    vals = {}
+    want = tuple(launch.get(k) for k in ("vec", "ty", "kchunk", "order"))
     for line in open(TRAFFIC_FILE):
-        if line.startswith("traffic:"):
-            for kv in line[len("traffic:"):].split():
-                k, _, v = kv.partition("=")
-                vals[k] = v
-    try:
-        if vals.get("workload") != workload_key:
-            return None
-        shape = tuple(int(x) for x in vals["launch"].split(","))
-        if shape != tuple(launch[k] for k in ("vec", "ty", "kchunk", "order")):
-            return None
-        return float(vals["bytes"])
-    except (KeyError, ValueError, TypeError):
-        return None
+        if not line.startswith("traffic:"):
+            continue
+        vals = dict(kv.partition("=")[::2] for kv in line[len("traffic:"):].split())
+        try:
+            if vals.get("workload") != workload_key:
+                continue
+            if tuple(int(x) for x in vals["launch"].split(",")) == want:
+                return float(vals["bytes"])
+        except (KeyError, ValueError, TypeError):
+            continue
+    return None
 
 
 def copy_probe():
@@ -211,6 +218,101 @@ def sweep_bench(n_designs=256, steps=4000, seq_sample=12):
     }
 
 
+def strip_bench(cpu_steps=60):
+    """2-D TM strip (540 x 290 total cells, FP64, 3000 steps with an
+    11-frequency line monitor): the reference-arithmetic ('compat') and the
+    textbook ('yee' + CPML) step on the GPU, and the O0 NumPy restatement of
+    the reference's step on one host core for a bounded number of steps."""
+    import numpy as np
+    import torch
+
+    from oracle import compat as O0
+    from prismo_amd import workloads as W
+
+    out = {}
+    for scheme in ("compat", "yee"):
+        sim, mon, steps = W.strip_2d(scheme=scheme)
+        sim.run_steps(10)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sim.run_steps(steps)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        nx, ny, _ = sim.grid.dimensions
+        out[scheme] = {"gcells_per_s": round(nx * ny * steps / el / 1e9, 4),
+                       "us_per_step": round(el / steps * 1e6, 2), "steps": steps}
+        del sim
+    sim, mon, _ = W.strip_2d(scheme="compat")
+    nx, ny, nz = sim.grid.dimensions
+    eps = W._strip_eps()["eps_rel"]
+    shp = (nx, ny, nz)
+    C = dict(zip(("Ca", "Cb", "Da", "Db"),
+                 O0.coefficients(eps.reshape(shp), np.ones(shp), np.zeros(shp), np.zeros(shp),
+                                 sim.dt)))
+    F = O0.zeros(nx, ny, nz, True)
+    src = sim.sources[0]
+    t0 = time.perf_counter()
+    O0.run(F, C, sim.grid.spacing, True, sim.dt, cpu_steps,
+           sources=[lambda F_, t: src.update_fields(F_, t, sim.dt)])
+    el = time.perf_counter() - t0
+    out["cpu_o0"] = {"gcells_per_s": round(nx * ny * cpu_steps / el / 1e9, 5),
+                     "us_per_step": round(el / cpu_steps * 1e6, 1), "steps": cpu_steps,
+                     "cores": 1, "kind": "port"}
+    out["cells"] = nx * ny
+    out["workload"] = ("strip_2d: 540 x 290 total cells (20-cell layers), 2-D TM FP64, "
+                       "500 nm Si strip, 2 um line source 193 THz, 11-frequency line monitor")
+    return out
+
+
+def waveguide_bench(steps=None):
+    """Dispersive 3-D waveguide (480 x 240 x 160 cells, Lorentz Si core,
+    Drude pad, grid-mode source, 64-frequency DFT plane): FP32 against FP64
+    over the full run; relative L2 of the plane's spectra and of the flux."""
+    import numpy as np
+    import torch
+
+    from prismo_amd import workloads as W
+
+    steps = steps or W.WG_STEPS
+    res, spec = {}, {}
+    for name, dt in (("fp32", np.float32), ("fp64", np.float64)):
+        sim, mon, mode, _ = W.waveguide_3d(dtype=dt)
+        tuned = sim.autotune() if dt == np.float32 else {"fused": False}
+        sim.run_steps(2)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sim.run_steps(steps - 2)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        cells = int(np.prod(W.WG_DIMS))
+        flux = W.waveguide_flux(mon)
+        spec[name] = {c: np.stack([np.squeeze(mon.get_frequency_data(c, f))
+                                   for f in mon.frequencies]) for c in ("Ey", "Ez", "Hy", "Hz")}
+        res[name] = {"gcells_per_s": round(cells * (steps - 2) / el / 1e9, 3),
+                     "ms_per_step": round(el / (steps - 2) * 1e3, 4), "flux": flux,
+                     "fused": bool(tuned.get("fused", False)),
+                     "grid_neff": float(sim.sources[0].grid_neff)}
+        cols = sum(int(np.prod(a.shape[1:])) for a in spec[name].values())
+        print(f"[bench] waveguide {name}: {el:.2f} s for {steps - 2} steps", file=sys.stderr,
+              flush=True)
+        del sim, mon
+        torch.cuda.empty_cache()
+    a, b = spec["fp32"], spec["fp64"]
+    num = sum(np.sum(np.abs(a[c] - b[c]) ** 2, axis=(1, 2)) for c in a)
+    den = sum(np.sum(np.abs(b[c]) ** 2, axis=(1, 2)) for c in a)
+    rel = np.sqrt(num / den)
+    fr = np.abs(res["fp32"]["flux"] / res["fp64"]["flux"] - 1)
+    return {
+        "workload": "waveguide_3d: 480 x 240 x 160 cells, Lorentz Si core + Drude pad, "
+                    "TE grid-mode source, 64-frequency Ey/Ez/Hy/Hz plane",
+        "steps": steps, "plane_columns": cols, "frequencies": 64,
+        "fp32": {k: v for k, v in res["fp32"].items() if k != "flux"},
+        "fp64": {k: v for k, v in res["fp64"].items() if k != "flux"},
+        "spectra_rel_l2_fp32_vs_fp64_max": float(rel.max()),
+        "flux_rel_err_fp32_vs_fp64_max": float(fr.max()),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -224,6 +326,8 @@ def main():
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--surrogate", action="store_true")
     ap.add_argument("--sweep-only", action="store_true")
+    ap.add_argument("--no-strip", action="store_true")
+    ap.add_argument("--no-waveguide", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only; runs on several GPUs are not supported "
@@ -296,6 +400,8 @@ def main():
     del sim, eng
     torch.cuda.empty_cache()
     sweep = None if args.no_sweep else sweep_bench()
+    strip = None if args.no_strip else strip_bench()
+    waveguide = None if args.no_waveguide else waveguide_bench()
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] timing the CPU oracle sample", file=sys.stderr, flush=True)
@@ -334,6 +440,8 @@ def main():
         },
         "cpu_baseline": cpu,
         "sweep": sweep,
+        "strip_2d": strip,
+        "waveguide_3d": waveguide,
     }
     print(json.dumps(line), flush=True)
 

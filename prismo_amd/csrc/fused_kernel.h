@@ -93,8 +93,8 @@ struct AdeOut {
 };
 
 template <typename T>
-__device__ __noinline__ AdeOut<T> fade(const KAde* tab, int n_ade, int comp, int i, int j,
-                                       int k, T curl, T e) {
+__device__ __forceinline__ AdeOut<T> fade(const KAde* tab, int n_ade, int comp, int i, int j,
+                                          int k, T curl, T e) {
 #pragma unroll 1
   for (int b = 0; b < n_ade; ++b) {
     const KAde& a = tab[b];
@@ -197,7 +197,7 @@ struct FusedLds {
 
 // One tile (TY rows x one z-chunk); b = the workgroup's index within the
 // launch segment of its kind (slab tiles or interior tiles).
-template <typename T, int CK, int VEC, int W, int TY, bool YZ>
+template <typename T, int CK, int VEC, int W, int TY, bool YZ, bool ADE>
 __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kchunk,
                                            int ntile_y, int nchunk, int nblocks, int order,
                                            const TileBox& bx, int b,
@@ -500,8 +500,7 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
       const int szz = (YZ && pml) ? fslab(kk, nz, g.pml[4], g.pml[5]) : -1;
       const bool pml_any = lane_pml_x || sys >= 0 || szz >= 0;
       const T zbek = ze[0], zcek = ze[1], zikek = ze[2];
-      const bool ade = g.n_ade && j >= g.ade_j0 && j < g.ade_j1 && kk >= g.ade_k0 &&
-                       kk < g.ade_k1;
+      const bool ade = ADE && row_has_ade(g, j, kk);
       T ox[VEC], oy[VEC], oz[VEC];
 #pragma unroll
       for (int e = 0; e < VEC; ++e) {
@@ -603,8 +602,7 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
     const int p = kk & 1;
     CbRaw<T, CK> cbz[VEC];
     fcb<T, CK, VEC>(z, 2, o, cbz);
-    const bool ade = g.n_ade && j >= g.ade_j0 && j < g.ade_j1 && kk >= g.ade_k0 &&
-                     kk < g.ade_k1;
+    const bool ade = ADE && row_has_ade(g, j, kk);
     T oz[VEC], zero[VEC];
     fzero(zero);
 #pragma unroll
@@ -641,7 +639,7 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
 // interior tiles fill the CUs the slab tiles leave idle at the end.  seg is
 // a multiple of 8, so a workgroup's XCD (blockIdx % 8) is the same in its
 // segment's numbering.
-template <typename T, int CK, int VEC, int W, int TY>
+template <typename T, int CK, int VEC, int W, int TY, bool ADE>
 __global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, const KFz z,
                                                                  int kchunk, int ntile_y,
                                                                  int nchunk, int n_out,
@@ -650,10 +648,11 @@ __global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, 
   __shared__ FusedLds<T, VEC, W, TY> lds;
   const int b = blockIdx.x;
   if (b < seg)
-    fused_tile<T, CK, VEC, W, TY, true>(g, z, kchunk, ntile_y, nchunk, n_out, order, bx, b, lds);
+    fused_tile<T, CK, VEC, W, TY, true, ADE>(g, z, kchunk, ntile_y, nchunk, n_out, order, bx,
+                                             b, lds);
   else
-    fused_tile<T, CK, VEC, W, TY, false>(g, z, kchunk, ntile_y, nchunk, n_in, order, bx,
-                                         b - seg, lds);
+    fused_tile<T, CK, VEC, W, TY, false, ADE>(g, z, kchunk, ntile_y, nchunk, n_in, order, bx,
+                                              b - seg, lds);
 }
 
 template <typename T, int VEC, int W, int TY>
@@ -684,8 +683,17 @@ int launch_fused_one(const KGrid& g, const KFz& z, int kchunk, int order, hipStr
     const int n_out = ntile_y * nchunk - n_in;
     const int seg = ((n_out + 7) / 8) * 8;
     const int nwg = seg + ((n_in + 7) / 8) * 8;
-    hipLaunchKernelGGL((k_yee_fused<T, CK, VEC, W, TY>), dim3(nwg), dim3(64 * W * (TY + 1)), 0,
-                       s, g, z, kchunk, ntile_y, nchunk, n_out, n_in, seg, order, bx);
+    // plans with dispersive boxes take an instance with the ADE update
+    // inlined (rows that cut a box, checked per box); the others carry no
+    // ADE code at all
+    if (g.n_ade)
+      hipLaunchKernelGGL((k_yee_fused<T, CK, VEC, W, TY, true>), dim3(nwg),
+                         dim3(64 * W * (TY + 1)), 0, s, g, z, kchunk, ntile_y, nchunk, n_out,
+                         n_in, seg, order, bx);
+    else
+      hipLaunchKernelGGL((k_yee_fused<T, CK, VEC, W, TY, false>), dim3(nwg),
+                         dim3(64 * W * (TY + 1)), 0, s, g, z, kchunk, ntile_y, nchunk, n_out,
+                         n_in, seg, order, bx);
     return 0;
   } else {
     return -1;
