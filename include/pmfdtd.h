@@ -8,7 +8,7 @@
  * (src/prismo/sources/base.py:141-157) and monitors
  * (src/prismo/monitors/base.py:136-150) called from Python every step.  It has
  * no FFI; every entry point below is the native body that a 'mi355x' backend
- * binds in place of those Python calls (ctypes stub in INTEGRATION.md).
+ * binds in place of those Python calls (through ctypes: prismo_amd/_native.py).
  *
  * Conventions
  *  - Status: 0 = ok, > 0 = a hipError_t, < 0 = a PM_E* argument/config error.
