@@ -109,8 +109,17 @@ __device__ __forceinline__ AdeOut<T> fade(const KAde* tab, int n_ade, int comp, 
   return AdeOut<T>{e, false};
 }
 
-// Does row j of plane k cut any dispersive box? (same for every lane)
+// Linear index of cell (i, j, k) in box a's state arrays (box order, i fastest).
+__device__ __forceinline__ int64_t ade_index(const KAde& a, int i, int j, int k) {
+  return (int64_t)(i - a.i0) +
+         (int64_t)(a.i1 - a.i0) * ((int64_t)(j - a.j0) + (int64_t)(a.j1 - a.j0) * (k - a.k0));
+}
+
+// Does row j of plane k cut any dispersive box? (same for every lane)  The
+// union of the boxes' rows and planes (kernel arguments) rules most rows out
+// before the box table is read.
 __device__ __forceinline__ bool row_has_ade(const KGrid& g, int j, int k) {
+  if (j < g.ade_j0 || j >= g.ade_j1 || k < g.ade_k0 || k >= g.ade_k1) return false;
   for (int b = 0; b < g.n_ade; ++b) {
     const KAde& a = g.ade[b];
     if (j >= a.j0 && j < a.j1 && k >= a.k0 && k < a.k1) return true;
@@ -403,6 +412,42 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
         ze[t] = *zp<T>(z.bc, z.prof[2] + (uint32_t)t * z.pn[2], kq);
       }
     }
+    // (1b) dispersive rows: the box of each lane's cell per component and
+    //      its state (P, and Q for Lorentz), issued with the E-side loads so
+    //      the E update does not wait for them; boxes are scanned in order
+    //      and the first one holding the cell takes it
+    const bool ade_row = ADE && do_e && row_has_ade(g, j, kk);
+    int abox[3][VEC];
+    T ap[3][VEC], aq[3][VEC];
+    if constexpr (ADE) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) {
+          abox[c][e] = -1;
+          ap[c][e] = aq[c][e] = (T)0;
+        }
+      if (ade_row) {
+        for (int t = 0; t < g.n_ade; ++t) {
+          const KAde& a = g.ade[t];
+          if (j < a.j0 || j >= a.j1 || kk < a.k0 || kk >= a.k1) continue;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            if (a.comp != c) continue;
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+              const int i = x0 + e;
+              if (abox[c][e] < 0 && i >= a.i0 && i < a.i1) {
+                abox[c][e] = t;
+                const int64_t l = ade_index(a, i, j, kk);
+                ap[c][e] = ((const T*)a.p)[l];
+                if (a.kind == 1) aq[c][e] = ((const T*)a.q)[l];
+              }
+            }
+          }
+        }
+      }
+    }
     // (2) prefetch plane k+1 (plane kh again after the last plane: unused)
     load_plane(min(k + 1, kh), nxt);
     // (3) share E(n) of plane k
@@ -500,7 +545,6 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
       const int szz = (YZ && pml) ? fslab(kk, nz, g.pml[4], g.pml[5]) : -1;
       const bool pml_any = lane_pml_x || sys >= 0 || szz >= 0;
       const T zbek = ze[0], zcek = ze[1], zikek = ze[2];
-      const bool ade = ADE && row_has_ade(g, j, kk);
       T ox[VEC], oy[VEC], oz[VEC];
 #pragma unroll
       for (int e = 0; e < VEC; ++e) {
@@ -554,18 +598,20 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
           vy = vy + fcbv<T, CK>(g, cby[e]) * cy;
           vz = vz + fcbv<T, CK>(g, cbz[e]) * cz;
         }
-        if (ade) {
-          if (exok) {
-            const AdeOut<T> r = fade<T>(g.ade, g.n_ade, 0, i, j, kk, cx, exm[e]);
-            if (r.hit) vx = r.e;
-          }
-          if (eyok) {
-            const AdeOut<T> r = fade<T>(g.ade, g.n_ade, 1, i, j, kk, cy, eym[e]);
-            if (r.hit) vy = r.e;
-          }
-          if (ezok) {
-            const AdeOut<T> r = fade<T>(g.ade, g.n_ade, 2, i, j, kk, cz, ezm[e]);
-            if (r.hit) vz = r.e;
+        if constexpr (ADE) {
+          if (ade_row) {
+            // the box coefficients come from a uniform scan, each lane
+            // taking the box it found above
+            for (int t = 0; t < g.n_ade; ++t) {
+              const KAde& a = g.ade[t];
+              if (j < a.j0 || j >= a.j1 || kk < a.k0 || kk >= a.k1) continue;
+              if (a.comp == 0 && exok && abox[0][e] == t)
+                vx = ade_cell_state<T>(a, ade_index(a, i, j, kk), ap[0][e], aq[0][e], cx, exm[e]);
+              if (a.comp == 1 && eyok && abox[1][e] == t)
+                vy = ade_cell_state<T>(a, ade_index(a, i, j, kk), ap[1][e], aq[1][e], cy, eym[e]);
+              if (a.comp == 2 && ezok && abox[2][e] == t)
+                vz = ade_cell_state<T>(a, ade_index(a, i, j, kk), ap[2][e], aq[2][e], cz, ezm[e]);
+            }
           }
         }
         ox[e] = exok ? vx : (T)0;
