@@ -40,16 +40,13 @@ def port_power(monitor, normal: str = "x", direction: int = 1, frequencies=None)
     area = {"x": dy * (1.0 if g.is_2d else dz), "y": dx * (1.0 if g.is_2d else dz),
             "z": dx * dy}[normal]
     out = np.zeros(len(freqs))
-    for k, f in enumerate(freqs):
-        p = 0.0
-        for e, h, sgn in _NORMAL_PAIRS[normal]:
-            if e in monitor.components and h in monitor.components:
-                E = np.asarray(monitor.get_frequency_data(e, f)).ravel()
-                H = np.asarray(monitor.get_frequency_data(h, f)).ravel()
-                n = min(E.size, H.size)
-                p += sgn * 0.5 * np.real(np.sum(E[:n] * np.conj(H[:n]))) * area
-        out[k] = direction * p
-    return out
+    for e, h, sgn in _NORMAL_PAIRS[normal]:
+        if e in monitor.components and h in monitor.components:
+            E = np.stack([np.asarray(monitor.get_frequency_data(e, f)).ravel() for f in freqs])
+            H = np.stack([np.asarray(monitor.get_frequency_data(h, f)).ravel() for f in freqs])
+            n = min(E.shape[1], H.shape[1])
+            out += sgn * 0.5 * np.real(np.sum(E[:, :n] * np.conj(H[:, :n]), axis=1)) * area
+    return direction * out
 
 
 class SParameterAnalyzer:

@@ -16,7 +16,18 @@ MU0 = 4 * np.pi * 1e-7
 
 
 def cell_coefficients(eps_rel, mu_rel, sigma_e, sigma_m, dt):
-    """(Ca, Cb, Da, Db) at cell nodes, FP64 (solver.py:119-130)."""
+    """(Ca, Cb, Da, Db) at cell nodes, FP64 (solver.py:119-130).  For
+    lossless non-magnetic media the expressions reduce exactly (sigma = 0
+    gives (1 - 0) / (1 + 0) = 1 and x / (1 + 0) = x in IEEE arithmetic), so
+    that case skips the array passes that cannot change a bit."""
+    eps_rel = np.asarray(eps_rel)
+    if not np.any(sigma_e) and not np.any(sigma_m) and np.all(np.asarray(mu_rel) == 1.0):
+        shape = np.broadcast_shapes(eps_rel.shape, np.shape(mu_rel), np.shape(sigma_e),
+                                    np.shape(sigma_m))
+        Cb = np.broadcast_to(dt / (EPS0 * eps_rel), shape).astype(np.float64)
+        one = np.ones(shape)
+        Db = np.full(shape, dt / (MU0 * 1.0))
+        return one, Cb, one.copy(), Db
     eps = EPS0 * eps_rel
     sigma_dt_2eps = sigma_e * dt / (2 * eps)
     Ca = (1 - sigma_dt_2eps) / (1 + sigma_dt_2eps)

@@ -73,9 +73,9 @@ class PlaneWaveSource(Source):
         if not self.enabled or self._grid is None:
             return []
         amp = self.waveform.table(times)
-        s = self.direction_sign
-        terms = [SourceTerm(c, self._source_box[c], np.array([a * s for a in amp]))
-                 for c in self._e_components]
-        terms += [SourceTerm(c, self._source_box[c], np.array([(a / 377.0) * s for a in amp]))
+        s = float(self.direction_sign)
+        # elementwise IEEE operations: the same bits as the per-step scalars
+        terms = [SourceTerm(c, self._source_box[c], amp * s) for c in self._e_components]
+        terms += [SourceTerm(c, self._source_box[c], (amp / 377.0) * s)
                   for c in self._h_components]
         return terms

@@ -25,8 +25,26 @@ class Waveform:
 
     def table(self, times) -> np.ndarray:
         """Values at each time, one scalar call per entry (bitwise as the
-        per-step calls of the reference)."""
-        return np.array([self.evaluate(float(t)) for t in times], dtype=np.float64)
+        per-step calls of the reference).  Tables are remembered per
+        (waveform class, parameters, times), so the many equal sources of a
+        sweep evaluate the waveform once."""
+        t = np.ascontiguousarray(times, dtype=np.float64)
+        try:
+            key = (type(self), tuple(sorted(self.__dict__.items())), t.size, hash(t.tobytes()))
+            hash(key)
+        except TypeError:
+            key = None
+        if key is not None and key in _TABLES:
+            return _TABLES[key].copy()
+        out = np.array([self.evaluate(float(x)) for x in t], dtype=np.float64)
+        if key is not None:
+            if len(_TABLES) >= 16:
+                _TABLES.pop(next(iter(_TABLES)))
+            _TABLES[key] = out.copy()
+        return out
+
+
+_TABLES: dict = {}
 
 
 class ContinuousWave(Waveform):

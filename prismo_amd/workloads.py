@@ -176,7 +176,8 @@ def coupler_analyze(sim, params=None) -> dict:
     s31 = np.abs(sa.get_s_parameter(2, 0))
 
     def phase(m):
-        return np.angle(np.array([np.sum(m.get_frequency_data("Ez", f)) for f in freqs]))
+        ez = np.stack([np.asarray(m.get_frequency_data("Ez", f)).ravel() for f in freqs])
+        return np.angle(ez.sum(axis=1))
 
     ph_in = phase(ports["in"])
     return {"s21": s21, "s31": s31, "p_in": p_in,
