@@ -328,6 +328,7 @@ def main():
     ap.add_argument("--sweep-only", action="store_true")
     ap.add_argument("--no-strip", action="store_true")
     ap.add_argument("--no-waveguide", action="store_true")
+    ap.add_argument("--waveguide-only", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only; runs on several GPUs are not supported "
@@ -346,6 +347,9 @@ def main():
     pa.set_backend("mi355x", 0)
     if args.sweep_only:
         print(json.dumps({"sweep": sweep_bench()}), flush=True)
+        return
+    if args.waveguide_only:
+        print(json.dumps({"waveguide_3d": waveguide_bench()}), flush=True)
         return
     if args.surrogate:
         from prismo_amd.surrogate import loop
