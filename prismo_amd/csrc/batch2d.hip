@@ -85,20 +85,23 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
   const bool inx = i >= 0 && i < nx;
   const bool own = lane >= 1 && lane <= kStrip && inx;
   const int q = par, qn = par ^ 1;
-  const T* __restrict__ EzA = (const T*)D.ez[q];
-  const T* __restrict__ HxA = (const T*)D.hx[q];
-  const T* __restrict__ HyA = (const T*)D.hy[q];
-  T* __restrict__ EzB = (T*)D.ez[qn];
-  T* __restrict__ HxB = (T*)D.hx[qn];
-  T* __restrict__ HyB = (T*)D.hy[qn];
-  const T* __restrict__ psHxA = (const T*)D.psi_hx[q];
-  const T* __restrict__ psHyA = (const T*)D.psi_hy[q];
-  T* __restrict__ psHxB = (T*)D.psi_hx[qn];
-  T* __restrict__ psHyB = (T*)D.psi_hy[qn];
-  T* __restrict__ psEx = (T*)D.psi_ex;
-  T* __restrict__ psEy = (T*)D.psi_ey;
-  const T* __restrict__ prof = (const T*)D.prof;
-  const T* __restrict__ py = prof + 6 * (int64_t)nx;
+  // the design table holds generic pointers; they all address device
+  // memory, and saying so gives global (not flat) loads, whose counters the
+  // compiler can wait on per load
+  const gptr<const T> EzA = (gptr<const T>)D.ez[q];
+  const gptr<const T> HxA = (gptr<const T>)D.hx[q];
+  const gptr<const T> HyA = (gptr<const T>)D.hy[q];
+  const gptr<T> EzB = (gptr<T>)D.ez[qn];
+  const gptr<T> HxB = (gptr<T>)D.hx[qn];
+  const gptr<T> HyB = (gptr<T>)D.hy[qn];
+  const gptr<const T> psHxA = (gptr<const T>)D.psi_hx[q];
+  const gptr<const T> psHyA = (gptr<const T>)D.psi_hy[q];
+  const gptr<T> psHxB = (gptr<T>)D.psi_hx[qn];
+  const gptr<T> psHyB = (gptr<T>)D.psi_hy[qn];
+  const gptr<T> psEx = (gptr<T>)D.psi_ex;
+  const gptr<T> psEy = (gptr<T>)D.psi_ey;
+  const gptr<const T> prof = (gptr<const T>)D.prof;
+  const gptr<const T> py = prof + 6 * (int64_t)nx;
 
   // x profiles of this lane (identity outside the slabs)
   const bool xpml = inx && (i < D.pml[0] || i >= nx - D.pml[1]);
@@ -116,7 +119,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
   const int64_t trow = step - k.table_step0;
   const T se = (T)k.tab_e[trow];
   const T sh = (T)k.tab_h[trow];
-  T* Xrow = (T*)k.X + (step % k.sample_T) * (int64_t)k.ncols;
+  const gptr<T> Xrow = (gptr<T>)k.X + (step % k.sample_T) * (int64_t)k.ncols;
   // everything the row loop needs from the design table, read once: the
   // loop's stores could alias the table as far as the compiler knows
   const int pml_y0 = D.pml[2], pml_y1 = ny - D.pml[3];
@@ -126,7 +129,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
   const int sh0 = D.src_h[1], sh1 = D.src_h[2];
   const T cbs = (T)D.cb_scale;
   using CbT = typename RowE<T, CK>::CbT;
-  const CbT* __restrict__ cbp = (const CbT*)D.cb;
+  const gptr<const CbT> cbp = (gptr<const CbT>)D.cb;
   // sample lines at this lane's column (bit l: line l)
   unsigned lmask = 0;
   const int nl = D.n_lines;
@@ -135,40 +138,38 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
   if (!own) lmask = 0;
   const bool cbx = inx && i < nx - 1;
 
+  // The field, Cb and profile loads of the row loop are issued on every
+  // path, from a clamped in-range position whose value is masked where it
+  // is used (lanes outside the design, rows past the last), so the
+  // compiler's memory-counter waits do not cover the next row's prefetch.
+  const int ic = min(max(i, 0), nx - 1);
   auto load_h = [&](int j, RowH<T>& r) __attribute__((always_inline)) {
-    const int64_t o = (int64_t)j * sy + i;
-    const bool yp = j < pml_y0 || j >= pml_y1;
-    r.ez = 0; r.hx = 0; r.hy = 0; r.psx = 0; r.psy = 0;
-    if (inx) {
-      r.ez = EzA[o];
-      r.hx = HxA[o];
-      r.hy = HyA[o];
-      if (yp) r.psx = psHxA[o];
-    }
+    const int jc = min(j, ny - 1);
+    const int64_t o = (int64_t)jc * sy + ic;
+    r.ez = EzA[o];
+    r.hx = HxA[o];
+    r.hy = HyA[o];
+    // the CPML states exist in the slabs only: reading them elsewhere
+    // would add four full-grid streams
+    r.psx = 0;
+    r.psy = 0;
+    if (j < pml_y0 || j >= pml_y1) r.psx = psHxA[o];
     if (xpml) r.psy = psHyA[o];
-    r.b = 1; r.c = 0; r.ik = 1;
-    if (yp) {
-      r.b = py[3 * ny + j];
-      r.c = py[4 * ny + j];
-      r.ik = py[5 * ny + j];
-    }
+    r.b = py[3 * ny + jc];
+    r.c = py[4 * ny + jc];
+    r.ik = py[5 * ny + jc];
   };
   auto load_e = [&](int jj, RowE<T, CK>& r) __attribute__((always_inline)) {
-    const int64_t o = (int64_t)jj * sy + i;
-    const bool yp = jj < pml_y0 || jj >= pml_y1;
-    const bool valid = cbx && jj < ny - 1;
-    r.cb = 0; r.pex = 0; r.pey = 0;
-    if (valid) {
-      r.cb = cbp[o];
-      if (xpml) r.pex = psEx[o];
-      if (yp) r.pey = psEy[o];
-    }
-    r.b = 1; r.c = 0; r.ik = 1;
-    if (yp) {
-      r.b = py[jj];
-      r.c = py[ny + jj];
-      r.ik = py[2 * ny + jj];
-    }
+    const int jc = min(jj, ny - 1);
+    const int64_t o = (int64_t)jc * sy + ic;
+    r.cb = cbp[o];
+    r.pex = 0;
+    r.pey = 0;
+    if (xpml) r.pex = psEx[o];
+    if (jj < pml_y0 || jj >= pml_y1) r.pey = psEy[o];
+    r.b = py[jc];
+    r.c = py[ny + jc];
+    r.ik = py[2 * ny + jc];
   };
 
   const int r0 = w.r0, r1 = w.r1;
@@ -180,14 +181,15 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
   // loads row j+1 into `nxt` and the E row j into `enxt` first
   auto body = [&](int j, const RowH<T>& cur, RowH<T>& nxt, const RowE<T, CK>& ecur,
                   RowE<T, CK>& enxt) __attribute__((always_inline)) {
-    if (j < jend) load_h(j + 1, nxt);
-    if (j < jend) load_e(j, enxt);
+    load_h(j + 1, nxt);  // row jend + 1 (or the last row again) is not used
+    load_e(j, enxt);
     const int64_t o = (int64_t)j * sy + i;
     const bool store_h = own && j < r1;
     const bool yp = j < pml_y0 || j >= pml_y1;
+    const T ezc = inx ? cur.ez : (T)0;
 
     // Hx(i, j) -= Db * dEz/dy (backward difference, Ez(j = -1) = 0)
-    T dy = (cur.ez - ezp) * idy;
+    T dy = (ezc - ezp) * idy;
     if (yp) {
       const T ps = cur.b * cur.psx + cur.c * dy;
       if (store_h) psHxB[o] = ps;
@@ -196,8 +198,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
     const T hxn = (inx && i < nx - 1) ? cur.hx - db * dy : (T)0;
 
     // Hy(i, j) += Db * dEz/dx, Ez(i-1) from the lane below
-    const T ezl = __shfl_up(cur.ez, 1);
-    T dx = (cur.ez - ezl) * idx;
+    const T ezl = __shfl_up(ezc, 1);
+    T dx = (ezc - ezl) * idx;
     if (xpml) {
       const T ps = bhx * cur.psy + chx * dx;
       if (store_h) psHyB[o] = ps;
@@ -261,7 +263,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
         }
       }
     }
-    ezp = cur.ez;
+    ezp = ezc;
     hxp = hxn;
     hyp = hyn;
   };

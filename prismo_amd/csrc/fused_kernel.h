@@ -440,8 +440,8 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
               if (abox[c][e] < 0 && i >= a.i0 && i < a.i1) {
                 abox[c][e] = t;
                 const int64_t l = ade_index(a, i, j, kk);
-                ap[c][e] = ((const T*)a.p)[l];
-                if (a.kind == 1) aq[c][e] = ((const T*)a.q)[l];
+                ap[c][e] = ((gptr<const T>)a.p)[l];
+                if (a.kind == 1) aq[c][e] = ((gptr<const T>)a.q)[l];
               }
             }
           }

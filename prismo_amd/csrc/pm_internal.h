@@ -83,13 +83,20 @@ constexpr int kScratchAde = 1024;
 // One dispersive cell: new E from the old E and the curl of H, advancing the
 // polarisation state at linear index l of the box.  Drude boxes have kq = 0,
 // a3 = a4 = 0 and no Q array.
+// Global-memory pointer: pointers read from device tables are generic to
+// the compiler, which then emits flat accesses whose completion it cannot
+// count per access (they may be LDS); every such table here points to
+// device memory.
+template <typename T>
+using gptr = T __attribute__((address_space(1)))*;
+
 // The same with the state (p0 = P[l], q0 = Q[l]; q0 unused for Drude)
 // already loaded by the caller, which can then issue those loads early.
 template <typename T>
 __device__ __forceinline__ T ade_cell_state(const KAde& a, int64_t l, T p0, T q0, T curl, T e) {
-  T* P = (T*)a.p;
+  const gptr<T> P = (gptr<T>)a.p;
   if (a.kind == 1) {
-    T* Q = (T*)a.q;
+    const gptr<T> Q = (gptr<T>)a.q;
     const T en = (T)a.ke * e + (T)a.kp * p0 - (T)a.kq * q0 + (T)a.ks * curl;
     P[l] = (T)a.a0 * en + (T)a.a1 * e + (T)a.a2 * p0 + q0;
     Q[l] = (T)a.a3 * p0 + (T)a.a4 * e;
@@ -102,8 +109,8 @@ __device__ __forceinline__ T ade_cell_state(const KAde& a, int64_t l, T p0, T q0
 
 template <typename T>
 __device__ __forceinline__ T ade_cell(const KAde& a, int64_t l, T curl, T e) {
-  const T p0 = ((const T*)a.p)[l];
-  const T q0 = a.kind == 1 ? ((const T*)a.q)[l] : (T)0;
+  const T p0 = ((gptr<const T>)a.p)[l];
+  const T q0 = a.kind == 1 ? ((gptr<const T>)a.q)[l] : (T)0;
   return ade_cell_state<T>(a, l, p0, q0, curl, e);
 }
 
