@@ -117,11 +117,11 @@ __device__ __forceinline__ int64_t ade_index(const KAde& a, int i, int j, int k)
 
 // Does row j of plane k cut any dispersive box? (same for every lane)  The
 // union of the boxes' rows and planes (kernel arguments) rules most rows out
-// before the box table is read.
-__device__ __forceinline__ bool row_has_ade(const KGrid& g, int j, int k) {
+// before the box table (the workgroup's LDS copy) is read.
+__device__ __forceinline__ bool row_has_ade(const KGrid& g, const KAde* tab, int j, int k) {
   if (j < g.ade_j0 || j >= g.ade_j1 || k < g.ade_k0 || k >= g.ade_k1) return false;
   for (int b = 0; b < g.n_ade; ++b) {
-    const KAde& a = g.ade[b];
+    const KAde& a = tab[b];
     if (j >= a.j0 && j < a.j1 && k >= a.k0 && k < a.k1) return true;
   }
   return false;
@@ -210,8 +210,10 @@ template <typename T, int CK, int VEC, int W, int TY, bool YZ, bool ADE>
 __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kchunk,
                                            int ntile_y, int nchunk, int nblocks, int order,
                                            const TileBox& bx, int b,
-                                           FusedLds<T, VEC, W, TY>& lds) {
+                                           FusedLds<T, VEC, W, TY>& lds, const KAde* tab) {
   constexpr int RW = 64 * VEC * W;
+  using ABits = std::conditional_t<(12 * VEC > 32), uint64_t, uint32_t>;
+  static_assert(kMaxAde < 16, "box numbers are kept in 4 bits");
   auto& le = lds.le;
   auto& lh = lds.lh;
   const int per = (nblocks + 7) / 8;
@@ -412,24 +414,26 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
         ze[t] = *zp<T>(z.bc, z.prof[2] + (uint32_t)t * z.pn[2], kq);
       }
     }
-    // (1b) dispersive rows: the box of each lane's cell per component and
-    //      its state (P, and Q for Lorentz), issued with the E-side loads so
-    //      the E update does not wait for them; boxes are scanned in order
-    //      and the first one holding the cell takes it
-    const bool ade_row = ADE && do_e && row_has_ade(g, j, kk);
-    int abox[3][VEC];
+    // (1b) dispersive rows: the box of each lane's cell per component (boxes
+    //      scanned in order, the first one holding the cell takes it) from
+    //      the LDS table, then its state (P, and Q for Lorentz), issued with
+    //      the E-side loads so the E update does not wait for them.  No loop
+    //      here or in (6) issues a memory access: with a loop-dependent
+    //      count of accesses in flight the compiler would wait for all of
+    //      them, the plane prefetch included.
+    const bool ade_row = ADE && do_e && row_has_ade(g, tab, j, kk);
+    // box number + 1 of (component c, cell e) in 4 bits at 4 (c VEC + e);
+    // 0: no box
+    ABits abits = 0;
     T ap[3][VEC], aq[3][VEC];
     if constexpr (ADE) {
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int e = 0; e < VEC; ++e) {
-          abox[c][e] = -1;
-          ap[c][e] = aq[c][e] = (T)0;
-        }
+        for (int e = 0; e < VEC; ++e) ap[c][e] = aq[c][e] = (T)0;
       if (ade_row) {
         for (int t = 0; t < g.n_ade; ++t) {
-          const KAde& a = g.ade[t];
+          const KAde& a = tab[t];
           if (j < a.j0 || j >= a.j1 || kk < a.k0 || kk >= a.k1) continue;
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
@@ -437,15 +441,27 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
 #pragma unroll
             for (int e = 0; e < VEC; ++e) {
               const int i = x0 + e;
-              if (abox[c][e] < 0 && i >= a.i0 && i < a.i1) {
-                abox[c][e] = t;
-                const int64_t l = ade_index(a, i, j, kk);
-                ap[c][e] = ((gptr<const T>)a.p)[l];
-                if (a.kind == 1) aq[c][e] = ((gptr<const T>)a.q)[l];
-              }
+              const int sh = 4 * (c * VEC + e);
+              if (((abits >> sh) & 15) == 0 && i >= a.i0 && i < a.i1)
+                abits |= (ABits)(t + 1) << sh;
             }
           }
         }
+        // every lane loads: one without a box reads the first word of the
+        // device copy of the table (valid memory, value unused)
+        const gptr<const T> dummy = (gptr<const T>)g.ade;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int e = 0; e < VEC; ++e) {
+            const int t = (int)((abits >> (4 * (c * VEC + e))) & 15) - 1;
+            const KAde& a = tab[t < 0 ? 0 : t];
+            const uint32_t l = t >= 0 ? (uint32_t)ade_index(a, x0 + e, j, kk) : 0u;
+            const gptr<const T> pp = t >= 0 ? (gptr<const T>)a.p + l : dummy;
+            const gptr<const T> qq = (t >= 0 && a.kind == 1) ? (gptr<const T>)a.q + l : dummy;
+            ap[c][e] = *pp;
+            aq[c][e] = *qq;
+          }
       }
     }
     // (2) prefetch plane k+1 (plane kh again after the last plane: unused)
@@ -600,17 +616,21 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
         }
         if constexpr (ADE) {
           if (ade_row) {
-            // the box coefficients come from a uniform scan, each lane
-            // taking the box it found above
-            for (int t = 0; t < g.n_ade; ++t) {
-              const KAde& a = g.ade[t];
-              if (j < a.j0 || j >= a.j1 || kk < a.k0 || kk >= a.k1) continue;
-              if (a.comp == 0 && exok && abox[0][e] == t)
-                vx = ade_cell_state<T>(a, ade_index(a, i, j, kk), ap[0][e], aq[0][e], cx, exm[e]);
-              if (a.comp == 1 && eyok && abox[1][e] == t)
-                vy = ade_cell_state<T>(a, ade_index(a, i, j, kk), ap[1][e], aq[1][e], cy, eym[e]);
-              if (a.comp == 2 && ezok && abox[2][e] == t)
-                vz = ade_cell_state<T>(a, ade_index(a, i, j, kk), ap[2][e], aq[2][e], cz, ezm[e]);
+            // each lane reads the coefficients of the box it found above
+            const int tx = (int)((abits >> (4 * e)) & 15) - 1;
+            const int ty_ = (int)((abits >> (4 * (VEC + e))) & 15) - 1;
+            const int tz = (int)((abits >> (4 * (2 * VEC + e))) & 15) - 1;
+            if (exok && tx >= 0) {
+              const KAde& a = tab[tx];
+              vx = ade_cell_state<T>(a, ade_index(a, i, j, kk), ap[0][e], aq[0][e], cx, exm[e]);
+            }
+            if (eyok && ty_ >= 0) {
+              const KAde& a = tab[ty_];
+              vy = ade_cell_state<T>(a, ade_index(a, i, j, kk), ap[1][e], aq[1][e], cy, eym[e]);
+            }
+            if (ezok && tz >= 0) {
+              const KAde& a = tab[tz];
+              vz = ade_cell_state<T>(a, ade_index(a, i, j, kk), ap[2][e], aq[2][e], cz, ezm[e]);
             }
           }
         }
@@ -648,7 +668,7 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
     const int p = kk & 1;
     CbRaw<T, CK> cbz[VEC];
     fcb<T, CK, VEC>(z, 2, o, cbz);
-    const bool ade = ADE && row_has_ade(g, j, kk);
+    const bool ade = ADE && row_has_ade(g, tab, j, kk);
     T oz[VEC], zero[VEC];
     fzero(zero);
 #pragma unroll
@@ -669,7 +689,7 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
       else
         vz = ezm[e] + fcbv<T, CK>(g, cbz[e]) * cz;
       if (ade && ezok) {
-        const AdeOut<T> r = fade<T>(g.ade, g.n_ade, 2, i, j, kk, cz, ezm[e]);
+        const AdeOut<T> r = fade<T>(tab, g.n_ade, 2, i, j, kk, cz, ezm[e]);
         if (r.hit) vz = r.e;
       }
       oz[e] = ezok ? vz : (T)0;
@@ -692,13 +712,25 @@ __global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, 
                                                                  int n_in, int seg, int order,
                                                                  TileBox bx) {
   __shared__ FusedLds<T, VEC, W, TY> lds;
+  // dispersive instances: the box table copied to LDS once per workgroup,
+  // so the per-row scans read LDS instead of issuing global loads whose
+  // waits would also wait for the plane prefetch in flight
+  __shared__ KAde ade_tab[ADE ? kMaxAde : 1];
+  if constexpr (ADE) {
+    static_assert(sizeof(KAde) % 4 == 0, "KAde is copied as 32-bit words");
+    const int n = g.n_ade * (int)(sizeof(KAde) / 4);
+    const gptr<const uint32_t> src = (gptr<const uint32_t>)g.ade;
+    uint32_t* dst = (uint32_t*)ade_tab;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) dst[t] = src[t];
+    __syncthreads();
+  }
   const int b = blockIdx.x;
   if (b < seg)
     fused_tile<T, CK, VEC, W, TY, true, ADE>(g, z, kchunk, ntile_y, nchunk, n_out, order, bx,
-                                             b, lds);
+                                             b, lds, ade_tab);
   else
     fused_tile<T, CK, VEC, W, TY, false, ADE>(g, z, kchunk, ntile_y, nchunk, n_in, order, bx,
-                                              b - seg, lds);
+                                              b - seg, lds, ade_tab);
 }
 
 template <typename T, int VEC, int W, int TY>
