@@ -74,7 +74,12 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
   // the wave index is uniform: make that visible so the tile and design
   // descriptors are scalar loads
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  const int item = blockIdx.x * kWaves + wv;
+  // workgroups go round-robin over the 8 XCDs (b % 8); in each group of 64,
+  // XCD x takes work groups 8x .. 8x+7, so neighbouring strips (which read
+  // the same cache lines at their edges) share one L2
+  const int b = blockIdx.x, nb = gridDim.x;
+  const int wg = b < (nb & ~63) ? (b & ~63) + (b & 7) * 8 + ((b >> 3) & 7) : b;
+  const int item = wg * kWaves + wv;
   if (item >= k.n_work) return;  // whole wave
   const KWork w = k.work[item];
   const pm_batch_design& D = k.des[w.d];
@@ -138,10 +143,10 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
   if (!own) lmask = 0;
   const bool cbx = inx && i < nx - 1;
 
-  // The field, Cb and profile loads of the row loop are issued on every
-  // path, from a clamped in-range position whose value is masked where it
-  // is used (lanes outside the design, rows past the last), so the
-  // compiler's memory-counter waits do not cover the next row's prefetch.
+  // Every load of the row loop is issued on every path, from a clamped
+  // in-range position whose value is masked where it is used (lanes outside
+  // the design, rows past the last, cells outside a CPML slab), so the
+  // compiler's memory-counter waits do not cover the prefetched rows.
   const int ic = min(max(i, 0), nx - 1);
   auto load_h = [&](int j, RowH<T>& r) __attribute__((always_inline)) {
     const int jc = min(j, ny - 1);
@@ -149,12 +154,11 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
     r.ez = EzA[o];
     r.hx = HxA[o];
     r.hy = HyA[o];
-    // the CPML states exist in the slabs only: reading them elsewhere
-    // would add four full-grid streams
-    r.psx = 0;
-    r.psy = 0;
-    if (j < pml_y0 || j >= pml_y1) r.psx = psHxA[o];
-    if (xpml) r.psy = psHyA[o];
+    // the CPML states are used in the slabs only: elsewhere every lane
+    // reads element 0 (one cache line per wave, no extra HBM traffic) and
+    // the value is not used
+    r.psx = psHxA[(j < pml_y0 || j >= pml_y1) ? o : 0];
+    r.psy = psHyA[xpml ? o : 0];
     r.b = py[3 * ny + jc];
     r.c = py[4 * ny + jc];
     r.ik = py[5 * ny + jc];
@@ -163,10 +167,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
     const int jc = min(jj, ny - 1);
     const int64_t o = (int64_t)jc * sy + ic;
     r.cb = cbp[o];
-    r.pex = 0;
-    r.pey = 0;
-    if (xpml) r.pex = psEx[o];
-    if (jj < pml_y0 || jj >= pml_y1) r.pey = psEy[o];
+    r.pex = psEx[xpml ? o : 0];
+    r.pey = psEy[(jj < pml_y0 || jj >= pml_y1) ? o : 0];
     r.b = py[jc];
     r.c = py[ny + jc];
     r.ik = py[2 * ny + jc];
@@ -174,15 +176,19 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
 
   const int r0 = w.r0, r1 = w.r1;
   const int jend = min(r1, ny - 1);
+  const int elast = max(jend - 1, r0);  // last E row the tile updates (or r0)
   T ezp = (r0 >= 1 && inx) ? EzA[(int64_t)(r0 - 1) * sy + i] : (T)0;
   T hxp = 0, hyp = 0;
 
   // one row: H(n+1/2) at row j from `cur`, E(n+1) at row j-1 from `ecur`;
-  // loads row j+1 into `nxt` and the E row j into `enxt` first
+  // first loads row j+2 into `nxt` and the E row j+1 into `enxt` (two rows
+  // ahead: the loads of two rows are in flight while one is computed)
   auto body = [&](int j, const RowH<T>& cur, RowH<T>& nxt, const RowE<T, CK>& ecur,
                   RowE<T, CK>& enxt) __attribute__((always_inline)) {
-    load_h(j + 1, nxt);  // row jend + 1 (or the last row again) is not used
-    load_e(j, enxt);
+    // rows past the tile are not used: the clamp makes those loads re-read
+    // a row already in the cache instead of fetching the next tile's rows
+    load_h(min(j + 2, jend), nxt);
+    load_e(min(j + 1, elast), enxt);
     const int64_t o = (int64_t)j * sy + i;
     const bool store_h = own && j < r1;
     const bool yp = j < pml_y0 || j >= pml_y1;
@@ -212,7 +218,11 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
       const T hys = (srcH && j >= sh0 && j < sh1) ? hyn + sh : hyn;
       HyB[o] = hys;
       if (lmask) {
-        for (int l = 0; l < nl; ++l) {
+        // a constant trip count (lmask holds no bit at or past nl): stores
+        // in a loop of run-time length would make the compiler wait for
+        // every access in flight, the row prefetch included
+#pragma unroll
+        for (int l = 0; l < PM_BATCH_MAX_LINES; ++l) {
           if (!((lmask >> l) & 1u)) continue;
           const pm_batch_line L = D.line[l];
           if (L.comp != 0 && j >= L.j0 && j < L.j1)
@@ -255,7 +265,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
       if (own) {
         EzB[oe] = ezn;
         if (lmask) {
-          for (int l = 0; l < nl; ++l) {
+#pragma unroll
+          for (int l = 0; l < PM_BATCH_MAX_LINES; ++l) {
             if (!((lmask >> l) & 1u)) continue;
             const pm_batch_line L = D.line[l];
             if (L.comp == 0 && jj >= L.j0 && jj < L.j1) Xrow[L.col + (jj - L.j0)] = ezn;
@@ -268,14 +279,20 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
     hyp = hyn;
   };
 
-  RowH<T> ha, hb;
-  RowE<T, CK> ea, eb;
-  load_h(r0, ha);
-  ea = RowE<T, CK>{};
+  // three register sets in rotation: iteration j computes from set
+  // (j - r0) % 3 and loads into set (j - r0 + 2) % 3
+  RowH<T> h0, h1, h2;
+  RowE<T, CK> e0, e1, e2;
+  load_h(r0, h0);
+  load_h(min(r0 + 1, jend), h1);
+  e0 = RowE<T, CK>{};  // E row r0 - 1: not updated by this tile
+  load_e(r0, e1);
   for (int j = r0;;) {
-    body(j, ha, hb, ea, eb);
+    body(j, h0, h2, e0, e2);
     if (++j > jend) break;
-    body(j, hb, ha, eb, ea);
+    body(j, h1, h0, e1, e0);
+    if (++j > jend) break;
+    body(j, h2, h1, e2, e1);
     if (++j > jend) break;
   }
 }
