@@ -158,7 +158,7 @@ def sweep_bench(n_designs=256, steps=4000, seq_sample=12):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     sims = [W.coupler_sim(p, backend=W.host_backend()) for p in designs]
-    eng = BatchEngine(sims, dtype=np.float32, coefficient_storage="fp16", chunk=16,
+    eng = BatchEngine(sims, dtype=np.float32, coefficient_storage="fp16", chunk=8,
                       sample_T=128)
     t1 = time.perf_counter()
     eng.run_steps(steps, graph=True, timing=True)

@@ -38,7 +38,7 @@ def _torch():
 
 class BatchEngine:
     def __init__(self, sims, dtype=np.float32, coefficient_storage: str = "auto",
-                 chunk: int = 16, sample_T: int = 128, device: int = 0):
+                 chunk: int = 8, sample_T: int = 128, device: int = 0):
         torch = _torch()
         if not sims:
             raise ValueError("empty batch")

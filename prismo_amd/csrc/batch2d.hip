@@ -1,15 +1,18 @@
 // Batched 2-D TM stepping of many uneven grids (include/pmbatch.h).
 //
 // One launch advances every design by one step.  The work list holds one
-// tile per wave: (design, x0, r0, r1) = a strip of 62 owned x cells
-// [x0, x0+62) and the rows [r0, r1).  The wave's 64 lanes cover x0-1 ..
-// x0+62: lane 0 and lane 63 are halo lanes that only provide Ez(i-1) and
-// Hy(i+1) to their neighbours through lane shuffles.  The wave marches up
-// its rows once, carrying row j-1 in registers:
+// tile per wave: (design, x0, r0, r1) = a strip of 64 x cells [x0, x0+64),
+// x0 a multiple of 64 (rows are padded to 64 cells, so every row access of
+// a wave is two whole 128-byte lines), and the rows [r0, r1).  The wave
+// marches up its rows once, carrying row j-1 in registers:
 //   H(n+1/2) at row j   from Ez(j), Ez(j-1) (registers) and Ez(i-1) (shuffle);
-//   E(n+1)   at row j-1 from Hx(j), Hx(j-1), Hy(i), Hy(i+1) (registers/shuffle),
-// so every array is read once and written once per step (plus one halo row
-// per tile and two halo columns per strip).  Reads come from field set
+//   E(n+1)   at row j-1 from Hx(j), Hx(j-1), Hy(i), Hy(i+1) (registers/shuffle).
+// The two values from outside the strip come from one extra load per row
+// (lanes 0..3: Ez(x0-1), and Ez, Hy and the Hy CPML state at x0+64): lane 0
+// takes Ez(x0-1) as its left neighbour, and Hy(n+1/2) at x0+64, which lane
+// 63's E update needs, is recomputed from them with the arithmetic of the
+// next strip's lane 0.  So every array is read once and written once per
+// step (plus one halo row per tile).  Reads come from field set
 // `par`, writes go to set par^1, so neighbouring tiles never see a
 // half-updated value; the H-side CPML memory is ping-ponged for the same
 // reason, the E-side memory is touched only by the owner and stays in place.
@@ -30,7 +33,7 @@
 namespace pm {
 namespace {
 
-constexpr int kStrip = 62;   // owned cells per wave
+constexpr int kStrip = 64;   // cells per wave
 constexpr int kWaves = 4;    // waves per workgroup
 
 struct KWork {
@@ -58,7 +61,21 @@ struct KBatch {
 template <typename T>
 struct RowH {
   T ez, hx, hy, psx, psy, b, c, ik;
+  T halo;  // lanes 0..3: Ez(x0-1), Ez(x0+64), Hy(x0+64), Hy CPML state at x0+64
 };
+
+// Value of v in lane l (l uniform), for the whole wave.
+template <typename T>
+__device__ __forceinline__ T lane_value(T v, int l) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+  } else {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+    return __builtin_bit_cast(T, ((unsigned long long)hi << 32) | lo);
+  }
+}
 
 template <typename T, int CK>
 struct RowE {
@@ -86,9 +103,10 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
   const int64_t step = use_clock ? (*k.clock + s_rel) : step_arg;
   const int nx = D.nx, ny = D.ny;
   const int64_t sy = D.sy;
-  const int i = w.x0 - 1 + lane;
-  const bool inx = i >= 0 && i < nx;
-  const bool own = lane >= 1 && lane <= kStrip && inx;
+  const int x0 = w.x0;
+  const int i = x0 + lane;
+  const bool inx = i < nx;
+  const bool own = inx;
   const int q = par, qn = par ^ 1;
   // the design table holds generic pointers; they all address device
   // memory, and saying so gives global (not flat) loads, whose counters the
@@ -119,6 +137,17 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
     chx = prof[4 * nx + i];
     ikhx = prof[5 * nx + i];
   }
+  // the same for the cell right of the strip, x0+64 (uniform)
+  const int ir = x0 + kStrip;
+  const bool rin = ir < nx;
+  const int irc = min(ir, nx - 1);
+  const bool xpml_r = rin && (ir < D.pml[0] || ir >= nx - D.pml[1]);
+  T bhx_r = 1, chx_r = 0, ikhx_r = 1;
+  if (xpml_r) {
+    bhx_r = prof[3 * nx + ir];
+    chx_r = prof[4 * nx + ir];
+    ikhx_r = prof[5 * nx + ir];
+  }
   const T db = (T)D.db;
   const T idx = (T)k.idx, idy = (T)k.idy;
   const int64_t trow = step - k.table_step0;
@@ -147,10 +176,17 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
   // in-range position whose value is masked where it is used (lanes outside
   // the design, rows past the last, cells outside a CPML slab), so the
   // compiler's memory-counter waits do not cover the prefetched rows.
-  const int ic = min(max(i, 0), nx - 1);
+  const int ic = min(i, nx - 1);
+  // the halo load: lane 1 Ez, lane 2 Hy, lane 3 the Hy CPML state (element
+  // 0 outside the x slabs) at x0+64; every other lane Ez(x0-1) (element 0
+  // of the row for the first strip)
+  const gptr<const T> hbase = lane == 2 ? HyA : (lane == 3 ? psHyA : EzA);
+  const int hcol = (lane == 1 || lane == 2 || (lane == 3 && xpml_r)) ? irc : max(x0 - 1, 0);
+  const bool hrow = !(lane == 3 && !xpml_r);  // false: element 0
   auto load_h = [&](int j, RowH<T>& r) __attribute__((always_inline)) {
     const int jc = min(j, ny - 1);
     const int64_t o = (int64_t)jc * sy + ic;
+    r.halo = hbase[hrow ? (int64_t)jc * sy + hcol : 0];
     r.ez = EzA[o];
     r.hx = HxA[o];
     r.hy = HyA[o];
@@ -178,7 +214,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
   const int jend = min(r1, ny - 1);
   const int elast = max(jend - 1, r0);  // last E row the tile updates (or r0)
   T ezp = (r0 >= 1 && inx) ? EzA[(int64_t)(r0 - 1) * sy + i] : (T)0;
-  T hxp = 0, hyp = 0;
+  T hxp = 0, hyp = 0, hyp_r = 0;
 
   // one row: H(n+1/2) at row j from `cur`, E(n+1) at row j-1 from `ecur`;
   // first loads row j+2 into `nxt` and the E row j+1 into `enxt` (two rows
@@ -203,8 +239,11 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
     }
     const T hxn = (inx && i < nx - 1) ? cur.hx - db * dy : (T)0;
 
-    // Hy(i, j) += Db * dEz/dx, Ez(i-1) from the lane below
-    const T ezl = __shfl_up(ezc, 1);
+    // Hy(i, j) += Db * dEz/dx, Ez(i-1) from the lane below (lane 0: the
+    // halo value, zero left of the grid)
+    const T ezh = x0 > 0 ? lane_value(cur.halo, 0) : (T)0;
+    const T ezs = __shfl_up(ezc, 1);
+    const T ezl = lane == 0 ? ezh : ezs;
     T dx = (ezc - ezl) * idx;
     if (xpml) {
       const T ps = bhx * cur.psy + chx * dx;
@@ -212,6 +251,14 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
       dx = dx * ikhx + ps;
     }
     const T hyn = (inx && j < ny - 1) ? cur.hy + db * dx : (T)0;
+    // Hy(n+1/2) at x0+64 for lane 63's E update: the next strip's lane 0
+    // arithmetic on the halo values (not stored)
+    T dxr = (lane_value(cur.halo, 1) - lane_value(ezc, 63)) * idx;
+    if (xpml_r) {
+      const T ps = bhx_r * lane_value(cur.halo, 3) + chx_r * dxr;
+      dxr = dxr * ikhx_r + ps;
+    }
+    const T hyn_r = (rin && j < ny - 1) ? lane_value(cur.halo, 2) + db * dxr : (T)0;
 
     if (store_h) {
       HxB[o] = hxn;
@@ -235,7 +282,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
       // Ez(i, jj) += Cb (dHy/dx - dHx/dy) with the new H of rows jj, jj+1
       const int jj = j - 1;
       const int64_t oe = o - sy;
-      const T hyr = __shfl_down(hyp, 1);
+      const T hys1 = __shfl_down(hyp, 1);
+      const T hyr = lane == 63 ? hyp_r : hys1;
       const bool valid = cbx && jj < ny - 1;
       T tx = (hyr - hyp) * idx;
       if (xpml && valid) {
@@ -277,6 +325,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
     ezp = ezc;
     hxp = hxn;
     hyp = hyn;
+    hyp_r = hyn_r;
   };
 
   // three register sets in rotation: iteration j computes from set

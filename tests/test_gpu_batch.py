@@ -15,7 +15,10 @@ from tests.conftest import rel_l2
 
 pytestmark = pytest.mark.gpu
 D = 20e-9
-SIZES = [(70, 40), (130, 25), (45, 90)]
+# with 8-cell CPML: 86, 146, 61 and 68 cells along x; strips of 64 cells
+# give tails, a one-strip design and (68) a strip whose right neighbour
+# cell, x = 64, lies in the x CPML
+SIZES = [(70, 40), (130, 25), (45, 90), (52, 33)]
 FREQS = [187e12, 193e12, 200e12, 205e12]
 
 
@@ -68,19 +71,19 @@ def _drive(o1, ref, sim, n):
         ref.update(o1.F, tt, sim.dt)
 
 
-def _run(dtype, storage, steps, graph, T=32, t=8):
+def _run(dtype, storage, steps, graph, T=32, t=8, chunk=32):
     from prismo_amd.core.batch import BatchEngine
 
     built = [_design(nxp, nyp, t, dtype, 100 + k) for k, (nxp, nyp) in enumerate(SIZES)]
     sims = [b[0] for b in built]
-    eng = BatchEngine(sims, dtype=dtype, coefficient_storage=storage, chunk=32, sample_T=T)
+    eng = BatchEngine(sims, dtype=dtype, coefficient_storage=storage, chunk=chunk, sample_T=T)
     eng.run_steps(steps, graph=graph)
     return eng, built
 
 
-@pytest.mark.parametrize("steps,graph", [(150, False), (151, False), (150, True)])
-def test_batch_fp64_matches_oracle(mi355x, steps, graph):
-    eng, built = _run(np.float64, "auto", steps, graph)
+@pytest.mark.parametrize("steps,graph,chunk", [(150, False, 32), (151, False, 7), (150, True, 32)])
+def test_batch_fp64_matches_oracle(mi355x, steps, graph, chunk):
+    eng, built = _run(np.float64, "auto", steps, graph, chunk=chunk)
     for d, (sim, eps) in enumerate(built):
         o1, ref = _oracle(sim, eps, 8, eng.cb_host[d])
         _drive(o1, ref, sim, steps)
