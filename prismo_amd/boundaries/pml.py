@@ -67,13 +67,34 @@ def depth(n: int, t_lo: int, t_hi: int, half: bool) -> np.ndarray:
     return np.clip(rho, 0.0, 1.0)
 
 
+_PROFILES: dict = {}
+
+
 def axis_profiles(n: int, t_lo: int, t_hi: int, d: float, dt: float,
                   params: PMLParams) -> dict:
     """b, c, 1/kappa at half nodes (E terms) and nodes (H terms) for one axis.
 
     Indices outside both layers get b = 1, c = 0, 1/kappa = 1.  Each layer
-    uses its own thickness for sigma_max and rho.
+    uses its own thickness for sigma_max and rho.  The profiles depend on
+    the arguments only and are remembered per argument set (a sweep of many
+    designs shares them), as read-only arrays.
     """
+    key = (n, t_lo, t_hi, float(d), float(dt), params.sigma_max, params.kappa_max,
+           params.alpha_max, params.polynomial_order, params.reflection)
+    hit = _PROFILES.get(key)
+    if hit is not None:
+        return hit
+    if len(_PROFILES) > 4096:
+        _PROFILES.clear()
+    out = _axis_profiles(n, t_lo, t_hi, d, dt, params)
+    for a in out.values():
+        a.setflags(write=False)
+    _PROFILES[key] = out
+    return out
+
+
+def _axis_profiles(n: int, t_lo: int, t_hi: int, d: float, dt: float,
+                   params: PMLParams) -> dict:
     m = params.polynomial_order
     out = {}
     for half, tag in ((True, "e"), (False, "h")):

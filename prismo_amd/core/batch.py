@@ -69,10 +69,16 @@ class BatchEngine:
         self.chunk = int(chunk)
         self._keep: list = []
         self.step_count = 0
-        self.cb_host: list = []
+        self._cb_q: list = []   # per design: (stored Cb, FP16 scale or None)
         self._times_done = 0.0
         self.ctx = N.BatchContext(device, self.stream.cuda_stream)
         self._built = False
+
+    @property
+    def cb_host(self) -> list:
+        """Per design, the Ez-point Cb the kernel applies, FP64 (the FP16
+        values times their scale for FP16 storage)."""
+        return [c if sc is None else c.astype(np.float64) * sc for c, sc in self._cb_q]
 
     # ------------------------------------------------------------------ setup
     def _alloc(self, shape, dtype=None):
@@ -102,12 +108,12 @@ class BatchEngine:
             cb_pad = np.zeros((ny, sy), np.float16)
             cb_pad[: ny - 1, : nx - 1] = q.T
             des.cb_scale = scale
-            self.cb_host.append(q.astype(np.float64) * scale)
+            self._cb_q.append((q, scale))
         else:
             cb_pad = np.zeros((ny, sy), np.float64 if self.precision == N.FP64 else np.float32)
             cb_pad[: ny - 1, : nx - 1] = cb.T
             des.cb_scale = 1.0
-            self.cb_host.append(cb)
+            self._cb_q.append((cb, None))
         # CPML: x then y profiles, six arrays each
         cp = up.cpml
         th = cp.thickness if cp is not None else (0, 0, 0, 0, 0, 0)

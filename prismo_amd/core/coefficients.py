@@ -19,15 +19,18 @@ def cell_coefficients(eps_rel, mu_rel, sigma_e, sigma_m, dt):
     """(Ca, Cb, Da, Db) at cell nodes, FP64 (solver.py:119-130).  For
     lossless non-magnetic media the expressions reduce exactly (sigma = 0
     gives (1 - 0) / (1 + 0) = 1 and x / (1 + 0) = x in IEEE arithmetic), so
-    that case skips the array passes that cannot change a bit."""
+    that case skips the array passes that cannot change a bit, and Ca, Da
+    and Db, constants there, are read-only views of one value."""
     eps_rel = np.asarray(eps_rel)
-    if not np.any(sigma_e) and not np.any(sigma_m) and np.all(np.asarray(mu_rel) == 1.0):
+    if is_lossless_nonmagnetic(mu_rel, sigma_e, sigma_m):
         shape = np.broadcast_shapes(eps_rel.shape, np.shape(mu_rel), np.shape(sigma_e),
                                     np.shape(sigma_m))
-        Cb = np.broadcast_to(dt / (EPS0 * eps_rel), shape).astype(np.float64)
-        one = np.ones(shape)
-        Db = np.full(shape, dt / (MU0 * 1.0))
-        return one, Cb, one.copy(), Db
+        Cb = dt / (EPS0 * eps_rel)
+        if Cb.shape != shape or Cb.dtype != np.float64:
+            Cb = np.broadcast_to(Cb, shape).astype(np.float64)
+        one = np.broadcast_to(np.float64(1.0), shape)
+        Db = np.broadcast_to(np.float64(dt / (MU0 * 1.0)), shape)
+        return one, Cb, one, Db
     eps = EPS0 * eps_rel
     sigma_dt_2eps = sigma_e * dt / (2 * eps)
     Ca = (1 - sigma_dt_2eps) / (1 + sigma_dt_2eps)
@@ -69,6 +72,15 @@ def component_average(A: np.ndarray, comp: str, is_2d: bool) -> np.ndarray:
     raise ValueError(comp)
 
 
+def _all_equal(a, v) -> bool:
+    """np.all(a == v), reading one element of a constant view (all strides
+    zero, as the defaults of unset material arrays are)."""
+    a = np.asarray(a)
+    if a.size and all(st == 0 for st in a.strides):
+        return bool(a.flat[0] == v)
+    return bool(np.all(a == v))
+
+
 def is_lossless_nonmagnetic(mu_rel, sigma_e, sigma_m) -> bool:
     """True when Ca = Da = 1 exactly and Db is one constant."""
-    return bool(np.all(sigma_e == 0) and np.all(sigma_m == 0) and np.all(mu_rel == 1))
+    return _all_equal(sigma_e, 0) and _all_equal(sigma_m, 0) and _all_equal(mu_rel, 1)

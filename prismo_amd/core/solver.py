@@ -80,14 +80,16 @@ class MaxwellUpdater:
     def _initialize_materials(self, material_arrays) -> None:
         shape = self.grid.dimensions
         ma = material_arrays or {}
-        self.eps_rel = np.asarray(ma.get("eps_rel", np.ones(shape)), dtype=np.float64)
-        self.mu_rel = np.asarray(ma.get("mu_rel", np.ones(shape)), dtype=np.float64)
-        self.sigma_e = np.asarray(ma.get("sigma_e", np.zeros(shape)), dtype=np.float64)
-        self.sigma_m = np.asarray(ma.get("sigma_m", np.zeros(shape)), dtype=np.float64)
-        for name in ("eps_rel", "mu_rel", "sigma_e", "sigma_m"):
-            a = getattr(self, name)
-            if a.shape != shape:
-                setattr(self, name, np.broadcast_to(a, shape).astype(np.float64))
+        for name, default in (("eps_rel", 1.0), ("mu_rel", 1.0), ("sigma_e", 0.0),
+                              ("sigma_m", 0.0)):
+            if name in ma:
+                a = np.asarray(ma[name], dtype=np.float64)
+                if a.shape != shape:
+                    a = np.broadcast_to(a, shape).astype(np.float64)
+            else:
+                # a constant: a read-only view of one value, no per-cell storage
+                a = np.broadcast_to(np.float64(default), shape)
+            setattr(self, name, a)
 
     def _compute_update_coefficients(self) -> None:
         self.Ca, self.Cb, self.Da, self.Db = cell_coefficients(
