@@ -40,10 +40,18 @@ def port_power(monitor, normal: str = "x", direction: int = 1, frequencies=None)
     area = {"x": dy * (1.0 if g.is_2d else dz), "y": dx * (1.0 if g.is_2d else dz),
             "z": dx * dy}[normal]
     out = np.zeros(len(freqs))
+    stacked = frequencies is None and hasattr(monitor, "frequency_stack")
+
+    def spectra(c):
+        if stacked:
+            a = monitor.frequency_stack(c)
+            return a.reshape(a.shape[0], -1)
+        return np.stack([np.asarray(monitor.get_frequency_data(c, f)).ravel() for f in freqs])
+
     for e, h, sgn in _NORMAL_PAIRS[normal]:
         if e in monitor.components and h in monitor.components:
-            E = np.stack([np.asarray(monitor.get_frequency_data(e, f)).ravel() for f in freqs])
-            H = np.stack([np.asarray(monitor.get_frequency_data(h, f)).ravel() for f in freqs])
+            E = spectra(e)
+            H = spectra(h)
             n = min(E.shape[1], H.shape[1])
             out += sgn * 0.5 * np.real(np.sum(E[:, :n] * np.conj(H[:, :n]), axis=1)) * area
     return direction * out

@@ -38,6 +38,7 @@ class FieldMonitor(Monitor):
         self._time_data = {c: [] for c in self.components} if time_domain else {}
         self._time_points: list = []
         self._freq_data: dict = {}
+        self._freq_stack: dict = {}
         if self.frequencies:
             for c in self.components:
                 self._freq_data[c] = {f: None for f in self.frequencies}
@@ -47,10 +48,17 @@ class FieldMonitor(Monitor):
         super().initialize(grid)
         for c in self.components:
             self._component_shapes[c] = tuple(ix.size for ix in self._monitor_region[c])
+        self._freq_stack = {}
         if self.frequencies:
             for c in self.components:
-                for f in self.frequencies:
-                    self._freq_data[c][f] = np.zeros(self._component_shapes[c], dtype=np.complex128)
+                self._set_stack(c, np.zeros((len(self.frequencies),) + self._component_shapes[c],
+                                            dtype=np.complex128))
+
+    def _set_stack(self, c, stack) -> None:
+        """All frequencies of component c as one (F, ...) array; the
+        per-frequency entries are views of it."""
+        self._freq_stack[c] = stack
+        self._freq_data[c] = {f: stack[fi] for fi, f in enumerate(self.frequencies)}
 
     def update(self, fields, time: float, dt: float) -> None:
         if self.time_domain:
@@ -86,8 +94,10 @@ class FieldMonitor(Monitor):
     def _ingest_spectra(self, per_comp) -> None:
         for c, spec in per_comp.items():
             shape = self._component_shapes[c]
-            for fi, f in enumerate(self.frequencies):
-                self._freq_data[c][f] = columns_to_region(spec[fi], shape).astype(np.complex128)
+            nf = len(self.frequencies)
+            # rows of box-ordered columns (i fastest) -> (F, i, j(, k))
+            a = np.asarray(spec, dtype=np.complex128).reshape((nf,) + shape[::-1])
+            self._set_stack(c, a.transpose((0,) + tuple(range(len(shape), 0, -1))))
 
     # -- queries ---------------------------------------------------------------
     def get_time_data(self, component):
@@ -96,6 +106,15 @@ class FieldMonitor(Monitor):
         if component not in self.components:
             raise ValueError(f"Component {component} not recorded by this monitor")
         return np.array(self._time_points), np.array(self._time_data[component])
+
+    def frequency_stack(self, component) -> np.ndarray:
+        """Spectra of ``component`` at every monitor frequency, stacked:
+        (F,) + the region shape, in the order of ``frequencies``."""
+        if not self.frequencies:
+            raise ValueError("Frequency domain data not enabled for this monitor")
+        if component not in self.components:
+            raise ValueError(f"Component {component} not recorded by this monitor")
+        return self._freq_stack[component]
 
     def get_frequency_data(self, component, frequency):
         if not self.frequencies:

@@ -176,8 +176,8 @@ def coupler_analyze(sim, params=None) -> dict:
     s31 = np.abs(sa.get_s_parameter(2, 0))
 
     def phase(m):
-        ez = np.stack([np.asarray(m.get_frequency_data("Ez", f)).ravel() for f in freqs])
-        return np.angle(ez.sum(axis=1))
+        ez = m.frequency_stack("Ez")
+        return np.angle(ez.reshape(ez.shape[0], -1).sum(axis=1))
 
     ph_in = phase(ports["in"])
     return {"s21": s21, "s31": s31, "p_in": p_in,
