@@ -304,7 +304,10 @@ class BatchEngine:
         p.work, p.work_bytes = ws.data_ptr(), wsb
         self.plan = p
         self._arr = arr
-        torch.cuda.synchronize(self.device)   # buffers zeroed before the library's stream runs
+        # the library's stream runs after the fills and uploads queued on
+        # torch's stream (a stream-side wait: other engines' work in flight
+        # on the device is not waited for)
+        self.stream.wait_stream(torch.cuda.current_stream(self.device))
         self.ctx.configure(p, keep=[arr])
         self.table_len = max(nsteps, 1)
         self._built = True
@@ -314,6 +317,14 @@ class BatchEngine:
         """Advance every design by ``nsteps`` steps (one configuration per
         engine: the tables cover exactly these steps).  ``timing`` records
         HIP-event times of every step launch / flush / graph replay."""
+        self.start(nsteps, graph=graph, timing=timing)
+        self.finish()
+
+    def start(self, nsteps: int, graph: bool = True, timing: bool = False) -> None:
+        """Set up and queue the whole run of ``nsteps`` steps on the engine's
+        stream, returning without waiting for it; ``finish`` waits and
+        delivers the results.  The host is free meanwhile (a sweep driver
+        builds the next group of designs)."""
         if self._built:
             raise RuntimeError("a BatchEngine runs once; build a new one for another run")
         self._build(nsteps)
@@ -321,6 +332,15 @@ class BatchEngine:
         self.ctx.set_timing(timing)
         self.ctx.run(nsteps, graph=graph and T % 2 == 0)
         self.ctx.flush()
+        self._queued = nsteps
+
+    def finish(self) -> None:
+        """Wait for the run ``start`` queued; set every Simulation's step
+        count and time and fill its monitors."""
+        nsteps = getattr(self, "_queued", None)
+        if nsteps is None:
+            raise RuntimeError("finish() without start()")
+        self._queued = None
         self.ctx.sync()
         self.step_count = nsteps
         t = 0.0

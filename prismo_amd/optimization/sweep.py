@@ -8,11 +8,13 @@ combinations are the meshgrid of the parameter values (sweep.py:85-103),
 
 Added: ``ParameterSweep.from_samples`` for a list of parameter dicts (random
 design sets), and a batched back end.  With ``batched=BatchedRunner(...)``
-the sweep builds every design's Simulation with ``build``, advances all of
-them together on the GPU (``prismo_amd.core.batch.BatchEngine``, one launch
-per step over every design, in groups of ``max_batch``), then calls
-``analyze(sim, params)`` per design.  A design whose build or analysis
-raises is reported with status 'error' and does not stop the others.
+the sweep builds every design's Simulation with ``build``, advances them
+together on the GPU (``prismo_amd.core.batch.BatchEngine``, one launch per
+step over every design of a group of ``max_batch``), then calls
+``analyze(sim, params)`` per design.  Groups are pipelined: group g+1 is
+built and queued on its own stream while group g runs, and group g is
+analysed while group g+1 runs.  A design whose build or analysis raises is
+reported with status 'error' and does not stop the others.
 """
 
 from __future__ import annotations
@@ -108,22 +110,17 @@ class ParameterSweep:
         br = self.batched
         combos = self.parameter_combinations
         results: list = [None] * len(combos)
-        built = []
-        for k, p in enumerate(combos):
+
+        def fail(keys, e):
+            for k in keys:
+                results[k] = {"parameters": combos[k], "status": "error", "error": str(e)}
+
+        def complete(eng, group):
             try:
-                built.append((k, br.build(p)))
+                eng.finish()
             except Exception as e:
-                results[k] = {"parameters": p, "status": "error", "error": str(e)}
-        for g in range(0, len(built), br.max_batch):
-            group = built[g:g + br.max_batch]
-            try:
-                eng = BatchEngine([s for _, s in group], dtype=br.dtype,
-                                  coefficient_storage=br.coefficient_storage, **br.engine_kw)
-                eng.run_steps(br.steps, graph=br.graph)
-            except Exception as e:
-                for k, _ in group:
-                    results[k] = {"parameters": combos[k], "status": "error", "error": str(e)}
-                continue
+                fail([k for k, _ in group], e)
+                return
             for k, sim in group:
                 try:
                     r = dict(br.analyze(sim, combos[k]))
@@ -131,7 +128,30 @@ class ParameterSweep:
                     r["status"] = "success"
                     results[k] = r
                 except Exception as e:
-                    results[k] = {"parameters": combos[k], "status": "error", "error": str(e)}
+                    fail([k], e)
+
+        pending = None  # (engine, group) queued on the GPU, not yet analysed
+        for g0 in range(0, len(combos), br.max_batch):
+            group = []
+            for k in range(g0, min(g0 + br.max_batch, len(combos))):
+                try:
+                    group.append((k, br.build(combos[k])))
+                except Exception as e:
+                    fail([k], e)
+            eng = None
+            if group:
+                try:
+                    eng = BatchEngine([s for _, s in group], dtype=br.dtype,
+                                      coefficient_storage=br.coefficient_storage, **br.engine_kw)
+                    eng.start(br.steps, graph=br.graph)
+                except Exception as e:
+                    fail([k for k, _ in group], e)
+                    eng = None
+            if pending is not None:
+                complete(*pending)
+            pending = (eng, group) if eng is not None else None
+        if pending is not None:
+            complete(*pending)
         return results
 
     # ----------------------------------------------------------------- results

@@ -39,8 +39,11 @@ def test_batched_build_errors_do_not_stop_the_sweep(monkeypatch):
         def __init__(self, sims, **kw):
             ran["n"] = len(sims)
 
-        def run_steps(self, n, graph=True):
+        def start(self, n, graph=True):
             ran["steps"] = n
+
+        def finish(self):
+            pass
 
     monkeypatch.setattr(B, "BatchEngine", FakeEngine)
 
@@ -56,6 +59,49 @@ def test_batched_build_errors_do_not_stop_the_sweep(monkeypatch):
     assert ran == {"n": 2, "steps": 7}
     assert [r["status"] for r in res] == ["success", "error", "success"]
     assert res[2]["x"] == 2 and "bad design" in res[1]["error"]
+
+
+def test_batched_groups_are_pipelined(monkeypatch):
+    """Group g+1 is built and queued before group g is waited for and
+    analysed; a group whose run fails reports every design as an error."""
+    import prismo_amd.core.batch as B
+
+    log = []
+
+    class FakeEngine:
+        def __init__(self, sims, **kw):
+            self.ids = [s["a"] for s in sims]
+
+        def start(self, n, graph=True):
+            log.append(("start", tuple(self.ids)))
+
+        def finish(self):
+            log.append(("finish", tuple(self.ids)))
+            if 4 in self.ids:
+                raise RuntimeError("device error")
+
+    monkeypatch.setattr(B, "BatchEngine", FakeEngine)
+
+    def build(p):
+        log.append(("build", p["a"]))
+        return dict(p)
+
+    def analyze(sim, p):
+        log.append(("analyze", p["a"]))
+        return {"x": p["a"]}
+
+    sw = ParameterSweep.from_samples(
+        [{"a": k} for k in range(5)],
+        batched=BatchedRunner(build=build, analyze=analyze, steps=3, max_batch=2))
+    res = sw.run()
+    assert log == [("build", 0), ("build", 1), ("start", (0, 1)),
+                   ("build", 2), ("build", 3), ("start", (2, 3)),
+                   ("finish", (0, 1)), ("analyze", 0), ("analyze", 1),
+                   ("build", 4), ("start", (4,)),
+                   ("finish", (2, 3)), ("analyze", 2), ("analyze", 3),
+                   ("finish", (4,))]
+    assert [r["status"] for r in res] == ["success"] * 4 + ["error"]
+    assert "device error" in res[4]["error"]
 
 
 def test_s_parameters_and_touchstone(tmp_path):
