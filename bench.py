@@ -43,8 +43,9 @@ designs, FDTD judge of the top 64 and the uncertain ones) and prints its
 own JSON line.
 
 ``--copy-probe`` runs only a 256 MiB device copy (the counter calibration
-kernel); ``--launch vec,ty,kchunk,order`` fixes the fused launch shape
-instead of autotuning.
+kernel); ``--batch-probe`` only 8 eager steps of the 256-coupler batch (for
+counter runs on the batch kernel); ``--launch vec,ty,kchunk,order`` fixes
+the fused launch shape instead of autotuning.
 """
 
 from __future__ import annotations
@@ -104,6 +105,22 @@ def copy_probe():
         b.copy_(a)
     torch.cuda.synchronize()
     print(f"[bench] copy probe: 4 x {a.numel() * 4} B copied", file=sys.stderr, flush=True)
+
+
+def batch_probe(steps=8):
+    """The 256 coupler designs of the sweep, one batch (8-row tiles), a few
+    eager steps: one k_batch_tm launch per step for counter runs."""
+    import numpy as np
+    import torch
+
+    from prismo_amd import workloads as W
+    from prismo_amd.core.batch import BatchEngine
+
+    sims = [W.coupler_sim(p, backend=W.host_backend()) for p in W.coupler_designs(256)]
+    eng = BatchEngine(sims, dtype=np.float32, coefficient_storage="fp16", chunk=8, sample_T=128)
+    eng.run_steps(steps, graph=False)
+    torch.cuda.synchronize()
+    print(f"[bench] batch probe: {steps} steps of 256 designs", file=sys.stderr, flush=True)
 
 
 def cpu_baseline(n=256, steps=3):
@@ -348,6 +365,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--launch", type=str, default=None)
     ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--batch-probe", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--surrogate", action="store_true")
     ap.add_argument("--sweep-only", action="store_true")
@@ -361,6 +379,9 @@ def main():
         sys.exit(2)
     if args.copy_probe:
         copy_probe()
+        return
+    if args.batch_probe:
+        batch_probe()
         return
 
     import numpy as np
