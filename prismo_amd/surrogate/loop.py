@@ -77,18 +77,21 @@ class Objective:
             a + b >= self.min_total)
 
 
-def fdtd(designs, steps=W.COUPLER_STEPS, max_batch=1024, dtype=np.float32, storage="fp16"):
-    """Batched FDTD judge: coupler_analyze results per design."""
-    from ..core.batch import BatchEngine
+def fdtd(designs, steps=W.COUPLER_STEPS, max_batch=64, dtype=np.float32, storage="fp16"):
+    """Batched FDTD judge: coupler_analyze results per design, through the
+    pipelined batched sweep (groups of ``max_batch``: one group is built and
+    analysed on the host while the previous one steps on the GPU)."""
+    from ..optimization.sweep import BatchedRunner, ParameterSweep
 
-    out = []
-    for g in range(0, len(designs), max_batch):
-        sims = [W.coupler_sim(p, backend=W.host_backend()) for p in designs[g:g + max_batch]]
-        eng = BatchEngine(sims, dtype=dtype, coefficient_storage=storage)
-        eng.run_steps(steps, graph=True)
-        out += [W.coupler_analyze(s) for s in sims]
-        del eng
-    return out
+    runner = BatchedRunner(build=lambda p: W.coupler_sim(p, backend=W.host_backend()),
+                           analyze=lambda sim, p: W.coupler_analyze(sim), steps=steps,
+                           dtype=dtype, coefficient_storage=storage, max_batch=max_batch,
+                           engine_kw={"chunk": 8})
+    res = ParameterSweep.from_samples(designs, batched=runner).run()
+    bad = [r for r in res if r["status"] != "success"]
+    if bad:
+        raise RuntimeError(f"FDTD judge: {len(bad)} designs failed, first: {bad[0]['error']}")
+    return res
 
 
 def run(n_train=4096, n_screen=65536, top_k=64, tau=0.05, members=4, epochs=150,
