@@ -28,6 +28,8 @@ struct pm_ctx {
   int parity = 0;
   KAux aux2{};  // source/sample view of the second buffer set
   std::vector<KAde> ade_host;  // staging of the ADE table upload
+  std::vector<KSbox> sbox_host;  // staging of the box table uploads
+  std::vector<KSrc> src_host;
   // timing
   bool timing = false;
   int timing_stride = 1;  // time the kernels of every timing_stride-th step
@@ -209,7 +211,9 @@ int validate(pm_ctx* c, const pm_plan& p) {
   }
   if (p.n_ade < 0 || p.n_ade > kMaxAde) return fail(c, PM_EINVAL, "too many ADE boxes");
   if (!p.scratch) return fail(c, PM_EINVAL, "scratch (PM_SCRATCH_BYTES) is required");
-  static_assert(kScratchAde + sizeof(KAde) * kMaxAde <= PM_SCRATCH_BYTES, "scratch layout");
+  static_assert(kScratchAde + sizeof(KAde) * kMaxAde <= kScratchSbox, "scratch layout");
+  static_assert(kScratchSbox + sizeof(KSbox) * kMaxSbox <= kScratchSrc, "scratch layout");
+  static_assert(kScratchSrc + sizeof(KSrc) * kMaxSrc <= PM_SCRATCH_BYTES, "scratch layout");
   for (int b = 0; b < p.n_ade; ++b) {
     const pm_ade_box& a = p.ade[b];
     if (a.comp < 0 || a.comp > 2 || !needE[a.comp] || (a.kind != 1 && a.kind != 2))
@@ -522,6 +526,43 @@ int pm_configure(pm_ctx* c, const pm_plan* plan) {
   for (int s = 0; s < p.n_sbox; ++s) {
     const pm_sample_box& b = p.sbox[s];
     x.sbox[s] = KSbox{b.comp, b.i0, b.i1, b.j0, b.j1, b.k0, b.k1, b.col0};
+  }
+  // box tables for the merged launches, uploaded (stream-ordered) into the
+  // caller's scratch; the host copies live in ctx until the next configure
+  auto cells = [](int i0, int i1, int j0, int j1, int k0, int k1) {
+    return (int64_t)std::max(i1 - i0, 0) * std::max(j1 - j0, 0) * std::max(k1 - k0, 0);
+  };
+  x.sbox_dev = (const KSbox*)((char*)p.scratch + kScratchSbox);
+  x.src_dev = (const KSrc*)((char*)p.scratch + kScratchSrc);
+  x.sbox_max = x.src_max = 0;
+  for (int s = 0; s < x.n_sbox; ++s) {
+    const KSbox& b = x.sbox[s];
+    x.sbox_max = std::max(x.sbox_max, cells(b.i0, b.i1, b.j0, b.j1, b.k0, b.k1));
+  }
+  x.src_merge = true;
+  for (int s = 0; s < x.n_src; ++s) {
+    const KSrc& a = x.src[s];
+    x.src_max = std::max(x.src_max, cells(a.i0, a.i1, a.j0, a.j1, a.k0, a.k1));
+    for (int t = 0; t < s; ++t) {
+      const KSrc& b = x.src[t];
+      if (a.comp == b.comp && a.i0 < b.i1 && b.i0 < a.i1 && a.j0 < b.j1 && b.j0 < a.j1 &&
+          a.k0 < b.k1 && b.k0 < a.k1)
+        x.src_merge = false;
+    }
+  }
+  c->sbox_host.assign(x.sbox, x.sbox + x.n_sbox);
+  c->src_host.assign(x.src, x.src + x.n_src);
+  if (x.n_sbox) {
+    int r = hip_check(c, hipMemcpyAsync((void*)x.sbox_dev, c->sbox_host.data(),
+                                        sizeof(KSbox) * x.n_sbox, hipMemcpyHostToDevice,
+                                        c->stream), "sample table upload");
+    if (r) return r;
+  }
+  if (x.n_src) {
+    int r = hip_check(c, hipMemcpyAsync((void*)x.src_dev, c->src_host.data(),
+                                        sizeof(KSrc) * x.n_src, hipMemcpyHostToDevice,
+                                        c->stream), "source table upload");
+    if (r) return r;
   }
   c->dft.clear();
   for (int s = 0; s < p.n_dft; ++s) {

@@ -79,6 +79,8 @@ struct KGrid {
 // scratch layout (caller-owned, PM_SCRATCH_BYTES)
 constexpr int kScratchFlags = 0;
 constexpr int kScratchAde = 1024;
+constexpr int kScratchSbox = 2048;  // sample boxes (merged sampling launch)
+constexpr int kScratchSrc = 2816;   // source boxes (merged source launch)
 
 // One dispersive cell: new E from the old E and the curl of H, advancing the
 // polarisation state at linear index l of the box.  Drude boxes have kq = 0,
@@ -125,6 +127,14 @@ struct KAux {
   int n_sbox, sample_T, ncols;
   KSbox sbox[kMaxSbox];
   void* X;
+  // device copies of the box tables (caller scratch) for one launch over
+  // all boxes of a kind; src_merge is false when two source boxes of one
+  // component overlap (their adds then keep the reference's order, one
+  // launch per box)
+  const KSbox* sbox_dev;
+  const KSrc* src_dev;
+  int64_t sbox_max, src_max;  // cells of the largest box
+  bool src_merge;
 };
 
 // One step's buffer pair for the fused kernel.

@@ -63,7 +63,13 @@ def test_compat_2d_bitwise(mi355x, lossy):
     mon = pa.FieldMonitor(center=(33 * D, 12 * D, 0), size=(0, 14 * D, 0),
                           components=["Ez", "Hy", "Hx"],
                           frequencies=[187e12, 193e12, 200e12], time_domain=True)
-    for s in (src, dip):
+    sources = [src, dip]
+    if lossy:
+        # a second dipole on the same Ez cell: overlapping boxes of one
+        # component are added box by box, in order (the others in one launch)
+        sources.append(pa.ElectricDipole(position=(30 * D, 6 * D, 0), polarization="z",
+                                         frequency=187e12, pulse_width=7e-15, amplitude=0.3))
+    for s in sources:
         sim.add_source(s)
     sim.add_monitor(mon)
     sim.run(150 * sim.dt * 0.9999)  # ceil -> 150 steps
@@ -71,7 +77,7 @@ def test_compat_2d_bitwise(mi355x, lossy):
 
     F, C = _o0_from_sim(sim, eps, sig)
     ref_mon = O0.RegionDFT({c: mon._monitor_region[c] for c in mon.components}, mon.frequencies)
-    srcs = [lambda F_, t, s=s: s.update_fields(F_, t, sim.dt) for s in (src, dip)]
+    srcs = [lambda F_, t, s=s: s.update_fields(F_, t, sim.dt) for s in sources]
     t = O0.run(F, C, g.spacing, True, sim.dt, 150, sources=srcs, monitors=[ref_mon])
     assert t == sim.current_time
     for c in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
