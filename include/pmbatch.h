@@ -53,7 +53,8 @@ typedef struct pm_batch_design {
   double cb_scale;           /* FP16 Cb = cb_scale * stored value (1 otherwise) */
   double db;                 /* dt / mu0 */
   const void* prof;          /* CPML profiles, field precision: x: be ce ike bh ch ikh (nx
-                                each), then y: the same six (ny each) */
+                                each), then y interleaved per row: 8 values per row j
+                                (be ce ike bh ch ikh, two unused), ny rows */
   int32_t src_e[3];          /* Ez line source: i, j0, j1 (j0 == j1: none) */
   int32_t src_h[3];          /* Hy line source: i, j0, j1 */
   pm_batch_line line[PM_BATCH_MAX_LINES];

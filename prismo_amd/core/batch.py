@@ -114,22 +114,26 @@ class BatchEngine:
             cb_pad[: ny - 1, : nx - 1] = cb.T
             des.cb_scale = 1.0
             self._cb_q.append((cb, None))
-        # CPML: x then y profiles, six arrays each
+        # CPML profiles: x as six arrays of nx (be ce ike bh ch ikh), then y
+        # interleaved per row, 8 values (the same six, two unused)
         cp = up.cpml
         th = cp.thickness if cp is not None else (0, 0, 0, 0, 0, 0)
         des.pml[0], des.pml[1], des.pml[2], des.pml[3] = th[0], th[1], th[2], th[3]
-        prof = np.zeros(6 * nx + 6 * ny)
-        prof[0:nx] = prof[2 * nx:3 * nx] = prof[3 * nx:4 * nx] = prof[5 * nx:6 * nx] = 1.0
-        o = 6 * nx
-        prof[o:o + ny] = prof[o + 2 * ny:o + 3 * ny] = prof[o + 3 * ny:o + 4 * ny] = 1.0
-        prof[o + 5 * ny:o + 6 * ny] = 1.0
+        keys = ("be", "ce", "ike", "bh", "ch", "ikh")
+        ident = (1.0, 0.0, 1.0, 1.0, 0.0, 1.0)
+        prof = np.zeros(6 * nx + 8 * ny)
+        px = prof[:6 * nx].reshape(6, nx)
+        py = prof[6 * nx:].reshape(ny, 8)
+        for k, v in enumerate(ident):
+            px[k] = v
+            py[:, k] = v
         if cp is not None:
             pr = cp.profiles(self.dt)
-            for a, (n, base) in enumerate(((nx, 0), (ny, 6 * nx))):
-                if pr[a] is None:
-                    continue
-                for k, key in enumerate(("be", "ce", "ike", "bh", "ch", "ikh")):
-                    prof[base + k * n: base + (k + 1) * n] = pr[a][key]
+            for k, key in enumerate(keys):
+                if pr[0] is not None:
+                    px[k] = pr[0][key]
+                if pr[1] is not None:
+                    py[:, k] = pr[1][key]
         self._host_cb.append(cb_pad)
         self._host_prof.append(prof)
         # monitors -> sample lines

@@ -60,9 +60,22 @@ struct KBatch {
 // so no wait lands on the loop's back edge.
 template <typename T>
 struct RowH {
-  T ez, hx, hy, psx, psy, b, c, ik;
+  T ez, hx, hy, psx, psy;
   T halo;  // lanes 0..3: Ez(x0-1), Ez(x0+64), Hy(x0+64), Hy CPML state at x0+64
 };
+
+// Element e of an array in global memory: a uniform base plus a 32-bit
+// byte offset, so one offset register serves every array of a row (the
+// host checks that a design's arrays fit 32-bit byte offsets).
+template <typename T>
+__device__ __forceinline__ T ldg(const void* base, uint32_t e) {
+  return *(gptr<const T>)((gptr<const char>)base + e * (uint32_t)sizeof(T));
+}
+
+template <typename T>
+__device__ __forceinline__ void stg(void* base, uint32_t e, T v) {
+  *(gptr<T>)((gptr<char>)base + e * (uint32_t)sizeof(T)) = v;
+}
 
 // Value of v in lane l (l uniform), for the whole wave.
 template <typename T>
@@ -81,7 +94,7 @@ template <typename T, int CK>
 struct RowE {
   using CbT = typename std::conditional<CK == PM_COEFF_CB_F16, unsigned short, T>::type;
   CbT cb;
-  T pex, pey, b, c, ik;
+  T pex, pey;
 };
 
 template <typename T, int CK>
@@ -111,20 +124,24 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
   // the design table holds generic pointers; they all address device
   // memory, and saying so gives global (not flat) loads, whose counters the
   // compiler can wait on per load
-  const gptr<const T> EzA = (gptr<const T>)D.ez[q];
-  const gptr<const T> HxA = (gptr<const T>)D.hx[q];
-  const gptr<const T> HyA = (gptr<const T>)D.hy[q];
-  const gptr<T> EzB = (gptr<T>)D.ez[qn];
-  const gptr<T> HxB = (gptr<T>)D.hx[qn];
-  const gptr<T> HyB = (gptr<T>)D.hy[qn];
-  const gptr<const T> psHxA = (gptr<const T>)D.psi_hx[q];
-  const gptr<const T> psHyA = (gptr<const T>)D.psi_hy[q];
-  const gptr<T> psHxB = (gptr<T>)D.psi_hx[qn];
-  const gptr<T> psHyB = (gptr<T>)D.psi_hy[qn];
-  const gptr<T> psEx = (gptr<T>)D.psi_ex;
-  const gptr<T> psEy = (gptr<T>)D.psi_ey;
+  const void* const EzA = D.ez[q];
+  const void* const HxA = D.hx[q];
+  const void* const HyA = D.hy[q];
+  void* const EzB = D.ez[qn];
+  void* const HxB = D.hx[qn];
+  void* const HyB = D.hy[qn];
+  const void* const psHxA = D.psi_hx[q];
+  const void* const psHyA = D.psi_hy[q];
+  void* const psHxB = D.psi_hx[qn];
+  void* const psHyB = D.psi_hy[qn];
+  void* const psEx = D.psi_ex;
+  void* const psEy = D.psi_ey;
   const gptr<const T> prof = (gptr<const T>)D.prof;
-  const gptr<const T> py = prof + 6 * (int64_t)nx;
+  // y profiles, 8 per row: read-only for the launch, so scalar loads when
+  // a row needs them (no vector registers held for them across the
+  // prefetch)
+  using CT = const T __attribute__((address_space(4)))*;
+  const CT pyr = (CT)(prof + 6 * (int64_t)nx);
 
   // x profiles of this lane (identity outside the slabs)
   const bool xpml = inx && (i < D.pml[0] || i >= nx - D.pml[1]);
@@ -163,7 +180,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
   const int sh0 = D.src_h[1], sh1 = D.src_h[2];
   const T cbs = (T)D.cb_scale;
   using CbT = typename RowE<T, CK>::CbT;
-  const gptr<const CbT> cbp = (gptr<const CbT>)D.cb;
+  const void* const cbp = D.cb;
   // sample lines at this lane's column (bit l: line l)
   unsigned lmask = 0;
   const int nl = D.n_lines;
@@ -180,40 +197,35 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
   // the halo load: lane 1 Ez, lane 2 Hy, lane 3 the Hy CPML state (element
   // 0 outside the x slabs) at x0+64; every other lane Ez(x0-1) (element 0
   // of the row for the first strip)
-  const gptr<const T> hbase = lane == 2 ? HyA : (lane == 3 ? psHyA : EzA);
+  const gptr<const T> hbase = (gptr<const T>)(lane == 2 ? HyA : (lane == 3 ? psHyA : EzA));
   const int hcol = (lane == 1 || lane == 2 || (lane == 3 && xpml_r)) ? irc : max(x0 - 1, 0);
   const bool hrow = !(lane == 3 && !xpml_r);  // false: element 0
+  const uint32_t syu = (uint32_t)sy;
   auto load_h = [&](int j, RowH<T>& r) __attribute__((always_inline)) {
     const int jc = min(j, ny - 1);
-    const int64_t o = (int64_t)jc * sy + ic;
+    const uint32_t o = (uint32_t)jc * syu + (uint32_t)ic;
     r.halo = hbase[hrow ? (int64_t)jc * sy + hcol : 0];
-    r.ez = EzA[o];
-    r.hx = HxA[o];
-    r.hy = HyA[o];
+    r.ez = ldg<T>(EzA, o);
+    r.hx = ldg<T>(HxA, o);
+    r.hy = ldg<T>(HyA, o);
     // the CPML states are used in the slabs only: elsewhere every lane
     // reads element 0 (one cache line per wave, no extra HBM traffic) and
     // the value is not used
-    r.psx = psHxA[(j < pml_y0 || j >= pml_y1) ? o : 0];
-    r.psy = psHyA[xpml ? o : 0];
-    r.b = py[3 * ny + jc];
-    r.c = py[4 * ny + jc];
-    r.ik = py[5 * ny + jc];
+    r.psx = ldg<T>(psHxA, (j < pml_y0 || j >= pml_y1) ? o : 0u);
+    r.psy = ldg<T>(psHyA, xpml ? o : 0u);
   };
   auto load_e = [&](int jj, RowE<T, CK>& r) __attribute__((always_inline)) {
     const int jc = min(jj, ny - 1);
-    const int64_t o = (int64_t)jc * sy + ic;
-    r.cb = cbp[o];
-    r.pex = psEx[xpml ? o : 0];
-    r.pey = psEy[(jj < pml_y0 || jj >= pml_y1) ? o : 0];
-    r.b = py[jc];
-    r.c = py[ny + jc];
-    r.ik = py[2 * ny + jc];
+    const uint32_t o = (uint32_t)jc * syu + (uint32_t)ic;
+    r.cb = ldg<CbT>(cbp, o);
+    r.pex = ldg<T>(psEx, xpml ? o : 0u);
+    r.pey = ldg<T>(psEy, (jj < pml_y0 || jj >= pml_y1) ? o : 0u);
   };
 
   const int r0 = w.r0, r1 = w.r1;
   const int jend = min(r1, ny - 1);
   const int elast = max(jend - 1, r0);  // last E row the tile updates (or r0)
-  T ezp = (r0 >= 1 && inx) ? EzA[(int64_t)(r0 - 1) * sy + i] : (T)0;
+  T ezp = (r0 >= 1 && inx) ? ldg<T>(EzA, (uint32_t)(r0 - 1) * syu + (uint32_t)i) : (T)0;
   T hxp = 0, hyp = 0, hyp_r = 0;
 
   // one row: H(n+1/2) at row j from `cur`, E(n+1) at row j-1 from `ecur`;
@@ -225,7 +237,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
     // a row already in the cache instead of fetching the next tile's rows
     load_h(min(j + 2, jend), nxt);
     load_e(min(j + 1, elast), enxt);
-    const int64_t o = (int64_t)j * sy + i;
+    const uint32_t o = (uint32_t)j * syu + (uint32_t)i;
     const bool store_h = own && j < r1;
     const bool yp = j < pml_y0 || j >= pml_y1;
     const T ezc = inx ? cur.ez : (T)0;
@@ -233,9 +245,10 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
     // Hx(i, j) -= Db * dEz/dy (backward difference, Ez(j = -1) = 0)
     T dy = (ezc - ezp) * idy;
     if (yp) {
-      const T ps = cur.b * cur.psx + cur.c * dy;
-      if (store_h) psHxB[o] = ps;
-      dy = dy * cur.ik + ps;
+      const CT q = pyr + 8 * j;
+      const T ps = q[3] * cur.psx + q[4] * dy;
+      if (store_h) stg<T>(psHxB, o, ps);
+      dy = dy * q[5] + ps;
     }
     const T hxn = (inx && i < nx - 1) ? cur.hx - db * dy : (T)0;
 
@@ -247,7 +260,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
     T dx = (ezc - ezl) * idx;
     if (xpml) {
       const T ps = bhx * cur.psy + chx * dx;
-      if (store_h) psHyB[o] = ps;
+      if (store_h) stg<T>(psHyB, o, ps);
       dx = dx * ikhx + ps;
     }
     const T hyn = (inx && j < ny - 1) ? cur.hy + db * dx : (T)0;
@@ -261,9 +274,9 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
     const T hyn_r = (rin && j < ny - 1) ? lane_value(cur.halo, 2) + db * dxr : (T)0;
 
     if (store_h) {
-      HxB[o] = hxn;
+      stg<T>(HxB, o, hxn);
       const T hys = (srcH && j >= sh0 && j < sh1) ? hyn + sh : hyn;
-      HyB[o] = hys;
+      stg<T>(HyB, o, hys);
       if (lmask) {
         // a constant trip count (lmask holds no bit at or past nl): stores
         // in a loop of run-time length would make the compiler wait for
@@ -281,21 +294,22 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
     if (j > r0) {
       // Ez(i, jj) += Cb (dHy/dx - dHx/dy) with the new H of rows jj, jj+1
       const int jj = j - 1;
-      const int64_t oe = o - sy;
+      const uint32_t oe = o - syu;
       const T hys1 = __shfl_down(hyp, 1);
       const T hyr = lane == 63 ? hyp_r : hys1;
       const bool valid = cbx && jj < ny - 1;
       T tx = (hyr - hyp) * idx;
       if (xpml && valid) {
         const T ps = bex * ecur.pex + cex * tx;
-        if (own) psEx[oe] = ps;
+        if (own) stg<T>(psEx, oe, ps);
         tx = tx * ikex + ps;
       }
       T ty = (hxn - hxp) * idy;
       if ((jj < pml_y0 || jj >= pml_y1) && valid) {
-        const T ps = ecur.b * ecur.pey + ecur.c * ty;
-        if (own) psEy[oe] = ps;
-        ty = ty * ecur.ik + ps;
+        const CT q = pyr + 8 * jj;
+        const T ps = q[0] * ecur.pey + q[1] * ty;
+        if (own) stg<T>(psEy, oe, ps);
+        ty = ty * q[2] + ps;
       }
       T cb;
       if constexpr (CK == PM_COEFF_CB_F16) {
@@ -311,7 +325,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
         if (srcE && jj >= se0 && jj < se1) ezn += se;
       }
       if (own) {
-        EzB[oe] = ezn;
+        stg<T>(EzB, oe, ezn);
         if (lmask) {
 #pragma unroll
           for (int l = 0; l < PM_BATCH_MAX_LINES; ++l) {
@@ -458,6 +472,10 @@ int validate(pm_batch* b, const pm_batch_plan& p) {
   for (int d = 0; d < p.n_design; ++d) {
     const pm_batch_design& D = p.design[d];
     if (D.nx < 2 || D.ny < 2 || D.sy < D.nx) return bfail(b, PM_EINVAL, "design %d: bad size", d);
+    // the kernel addresses a design's arrays with 32-bit byte offsets
+    if ((int64_t)D.ny * D.sy * (p.precision == PM_FP64 ? 8 : 4) >= (int64_t)1 << 32)
+      return bfail(b, PM_EINVAL, "design %d: %d x %lld cells exceed 32-bit offsets", d, D.ny,
+                   (long long)D.sy);
     for (int s = 0; s < 2; ++s)
       if (!D.ez[s] || !D.hx[s] || !D.hy[s] || !D.psi_hx[s] || !D.psi_hy[s])
         return bfail(b, PM_EINVAL, "design %d: missing field or CPML arrays", d);
