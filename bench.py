@@ -161,11 +161,12 @@ SWEEP_ALG_BYTES_PER_CELL = 26.0  # 2-D TM: Ez, Hx, Hy FP32 read + written, FP16 
 SWEEP_ALG_BYTES_PER_PML_CELL_AXIS = 16.0  # two FP32 psi read + written
 
 
-def sweep_bench(n_designs=256, steps=4000, seq_sample=12, group=64, roof_steps=512):
+def sweep_bench(n_designs=256, steps=4000, seq_sample=12, group=64, first=16, roof_steps=512):
     """Batched 256-coupler sweep through ParameterSweep's batched back end
     (FP32 fields, FP16 Cb, one launch per step over every design of a group
-    of ``group``, graph-replayed step loop, groups pipelined so the host
-    builds and analyses while the GPU steps), timed end to end, against
+    of ``group``, the first group ``first`` designs so the GPU starts early,
+    graph-replayed step loop, groups pipelined so the host builds and
+    analyses while the GPU steps), timed end to end, against
     sequential plain FP64 runs of an evenly spaced sample of the same
     designs, after an untimed warm-up sweep of 8 designs.  The roofline of
     the batch kernel is taken on one launch over all 256 designs
@@ -182,7 +183,7 @@ def sweep_bench(n_designs=256, steps=4000, seq_sample=12, group=64, roof_steps=5
     runner = BatchedRunner(build=lambda p: W.coupler_sim(p, backend=W.host_backend()),
                            analyze=lambda sim, p: W.coupler_analyze(sim), steps=steps,
                            dtype=np.float32, coefficient_storage="fp16", max_batch=group,
-                           engine_kw=kw)
+                           engine_kw=kw, first_batch=first)
     # untimed warm-up (kernel code loaded, first graph captured): 8 designs,
     # 64 steps, through the same path
     warm = BatchedRunner(build=runner.build, analyze=runner.analyze, steps=64,
@@ -200,7 +201,8 @@ def sweep_bench(n_designs=256, steps=4000, seq_sample=12, group=64, roof_steps=5
         raise RuntimeError(f"sweep: {len(bad)} designs failed, first: {bad[0].get('error')}")
     rate = n_designs / (t1 - t0)
     print(f"[bench] sweep: {n_designs} designs end to end in {t1 - t0:.2f} s "
-          f"(groups of {group}, pipelined)", file=sys.stderr, flush=True)
+          f"(first group {first}, then groups of {group}, pipelined)", file=sys.stderr,
+          flush=True)
 
     # the batch kernel's roofline: one launch per step over all the designs
     sims = [W.coupler_sim(p, backend=W.host_backend()) for p in designs]
@@ -243,6 +245,7 @@ def sweep_bench(n_designs=256, steps=4000, seq_sample=12, group=64, roof_steps=5
         "steps": steps,
         "designs_per_s": round(rate, 3),
         "group": group,
+        "first_group": first,
         "designs_per_s_gpu_only": round(n_designs / (step_ms * steps / 1e3), 3),
         "fp64_sequential_designs_per_s": round(seq_rate, 4),
         "fp64_sample": int(len(idx)),

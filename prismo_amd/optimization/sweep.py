@@ -40,7 +40,9 @@ class SweepParameter:
 
 @dataclass
 class BatchedRunner:
-    """build(params) -> Simulation (not run); analyze(sim, params) -> dict."""
+    """build(params) -> Simulation (not run); analyze(sim, params) -> dict.
+    Designs run in groups of ``max_batch``; ``first_batch`` (if set) sizes
+    the first group alone, so the GPU starts after a shorter build."""
     build: Callable[[dict], Any]
     analyze: Callable[[Any, dict], dict]
     steps: int
@@ -49,6 +51,7 @@ class BatchedRunner:
     max_batch: int = 256
     graph: bool = True
     engine_kw: dict = field(default_factory=dict)
+    first_batch: Optional[int] = None
 
 
 class ParameterSweep:
@@ -131,9 +134,13 @@ class ParameterSweep:
                     fail([k], e)
 
         pending = None  # (engine, group) queued on the GPU, not yet analysed
-        for g0 in range(0, len(combos), br.max_batch):
+        starts = [0]
+        first = br.first_batch if br.first_batch else br.max_batch
+        starts += list(range(min(first, len(combos)), len(combos), br.max_batch))
+        for gi, g0 in enumerate(starts):
+            g1 = starts[gi + 1] if gi + 1 < len(starts) else len(combos)
             group = []
-            for k in range(g0, min(g0 + br.max_batch, len(combos))):
+            for k in range(g0, g1):
                 try:
                     group.append((k, br.build(combos[k])))
                 except Exception as e:

@@ -79,14 +79,15 @@ class Objective:
 
 def fdtd(designs, steps=W.COUPLER_STEPS, max_batch=64, dtype=np.float32, storage="fp16"):
     """Batched FDTD judge: coupler_analyze results per design, through the
-    pipelined batched sweep (groups of ``max_batch``: one group is built and
-    analysed on the host while the previous one steps on the GPU)."""
+    pipelined batched sweep (groups of ``max_batch``, the first of 16: one
+    group is built and analysed on the host while the previous one steps on
+    the GPU)."""
     from ..optimization.sweep import BatchedRunner, ParameterSweep
 
     runner = BatchedRunner(build=lambda p: W.coupler_sim(p, backend=W.host_backend()),
                            analyze=lambda sim, p: W.coupler_analyze(sim), steps=steps,
                            dtype=dtype, coefficient_storage=storage, max_batch=max_batch,
-                           engine_kw={"chunk": 8})
+                           engine_kw={"chunk": 8}, first_batch=min(16, max_batch))
     res = ParameterSweep.from_samples(designs, batched=runner).run()
     bad = [r for r in res if r["status"] != "success"]
     if bad:

@@ -124,3 +124,28 @@ def test_s_parameters_and_touchstone(tmp_path):
     tau = 3e-15
     gd = compute_group_delay(f, np.exp(-1j * 2 * np.pi * f * tau))
     assert np.allclose(gd, tau)
+
+
+def test_batched_first_group_size(monkeypatch):
+    """``first_batch`` sizes the first group; the rest are ``max_batch``."""
+    import prismo_amd.core.batch as B
+
+    groups = []
+
+    class FakeEngine:
+        def __init__(self, sims, **kw):
+            groups.append([s["a"] for s in sims])
+
+        def start(self, n, graph=True):
+            pass
+
+        def finish(self):
+            pass
+
+    monkeypatch.setattr(B, "BatchEngine", FakeEngine)
+    sw = ParameterSweep.from_samples(
+        [{"a": k} for k in range(7)],
+        batched=BatchedRunner(build=dict, analyze=lambda s, p: {}, steps=1, max_batch=3,
+                              first_batch=1))
+    assert all(r["status"] == "success" for r in sw.run())
+    assert groups == [[0], [1, 2, 3], [4, 5, 6]]
