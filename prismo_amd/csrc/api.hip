@@ -574,6 +574,12 @@ int pm_configure(pm_ctx* c, const pm_plan* plan) {
     c->aux2.F[a] = p.E2[a];
     c->aux2.F[3 + a] = p.H2[a];
   }
+  // the uploads read host staging that the next configure overwrites:
+  // wait for them here (configure is set-up, not the step loop)
+  if (p.n_ade || x.n_sbox || x.n_src) {
+    int r = hip_check(c, hipStreamSynchronize(c->stream), "table upload sync");
+    if (r) return r;
+  }
   c->fused_ok = fused_eligible(p) &&
                 fused_addressable(c->g, fused_bufs(p, true), p.precision == PM_FP64 ? 8 : 4) &&
                 fused_addressable(c->g, fused_bufs(p, false), p.precision == PM_FP64 ? 8 : 4);
