@@ -168,7 +168,8 @@ def sweep_bench(n_designs=256, steps=4000, seq_sample=12, group=64, first=16, ro
     graph-replayed step loop, groups pipelined so the host builds and
     analyses while the GPU steps), timed end to end, against
     sequential plain FP64 runs of an evenly spaced sample of the same
-    designs, after an untimed warm-up sweep of 8 designs.  The roofline of
+    designs, after an untimed 64-step warm-up sweep of one group of each
+    size.  The roofline of
     the batch kernel is taken on one launch over all 256 designs
     (``roof_steps`` steps, HIP events)."""
     import numpy as np
@@ -184,12 +185,13 @@ def sweep_bench(n_designs=256, steps=4000, seq_sample=12, group=64, first=16, ro
                            analyze=lambda sim, p: W.coupler_analyze(sim), steps=steps,
                            dtype=np.float32, coefficient_storage="fp16", max_batch=group,
                            engine_kw=kw, first_batch=first)
-    # untimed warm-up (kernel code loaded, first graph captured): 8 designs,
-    # 64 steps, through the same path
+    # untimed warm-up through the same path with the timed run's group
+    # shapes (a group of `first`, then one of `group`), 64 steps: kernel code
+    # loaded, graphs captured, device memory of those sizes in torch's pool
     warm = BatchedRunner(build=runner.build, analyze=runner.analyze, steps=64,
-                         dtype=np.float32, coefficient_storage="fp16", max_batch=4,
-                         engine_kw=kw)
-    ParameterSweep.from_samples(designs[:8], batched=warm).run()
+                         dtype=np.float32, coefficient_storage="fp16", max_batch=group,
+                         engine_kw=kw, first_batch=first)
+    ParameterSweep.from_samples(designs[:first + group], batched=warm).run()
     sweep = ParameterSweep.from_samples(designs, batched=runner)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
