@@ -39,7 +39,7 @@ FP32 and FP64 with the relative L2 difference of the plane's spectra.
 
 ``--surrogate`` runs only the surrogate-screened coupler search (4096
 FDTD training spectra, a 4-member BF16 MLP ensemble, 65536 screened
-designs, FDTD judge of the top 64 and the uncertain ones) and prints its
+designs, FDTD check of the top 64 and the uncertain ones) and prints its
 own JSON line.
 
 ``--copy-probe`` runs only a 256 MiB device copy (the counter calibration

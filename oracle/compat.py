@@ -1,7 +1,7 @@
 """O0 -- CPU restatement of the reference's time-step arithmetic.
 
 TEST INFRASTRUCTURE ONLY.  Imported by tests/, __graft_entry__.smoke() and
-the cpu_baseline leg of bench.py as the checker; never by the product.
+the cpu_baseline leg of bench.py, to check and time; never by the product.
 
 Follows, operation for operation:
   src/prismo/core/solver.py:113-133   Ca/Cb/Da/Db           (coefficients)

@@ -1,6 +1,6 @@
 """O3 -- CPU restatement of the reference's shape rasterisation.
 
-TEST INFRASTRUCTURE ONLY.  Imported by tests/ as the checker of the device
+TEST INFRASTRUCTURE ONLY.  Imported by tests/ to check the device
 rasteriser and coefficient build (include/pmgeom.h); never by the product.
 
 Follows src/prismo/geometry/shapes.py: every shape tests points with the

@@ -1,6 +1,6 @@
 """O2 -- CPU restatement of the surrogate MLP's forward and backward passes.
 
-TEST INFRASTRUCTURE ONLY.  Imported by tests/ as the checker of the device
+TEST INFRASTRUCTURE ONLY.  Imported by tests/ to check the device
 kernels (include/pmmlp.h); never by the product.
 
 The reference has no learned model; this is the textbook dense network the

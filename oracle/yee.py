@@ -1,7 +1,7 @@
 """O1 -- CPU textbook Yee oracle with CPML, coupled ADE and region DFT.
 
 TEST INFRASTRUCTURE ONLY.  Imported by tests/, __graft_entry__.smoke() and
-the cpu_baseline leg of bench.py as the checker; never by the product.
+the cpu_baseline leg of bench.py, to check and time; never by the product.
 
 Scheme (stated here, pinned by the physics tests in tests/test_oracle_*):
   * staggering of src/prismo/core/solver.py:173-175,261-263 (E on faces, H on

@@ -8,7 +8,7 @@
 3. ``n_screen`` seeded candidates are scored by the ensemble; the top ``k``
    by the design objective and every candidate whose predicted objective
    is uncertain (ensemble spread above ``tau`` near the acceptance band)
-   are re-run by batched FDTD, the judge.  Only designs the judge accepts
+   are re-run by batched FDTD, which decides.  Only designs the FDTD runs accept
    count.
 
 Objective (a 3 dB coupler at 193 THz): |S21|^2 and |S31|^2 both within
@@ -78,7 +78,7 @@ class Objective:
 
 
 def fdtd(designs, steps=W.COUPLER_STEPS, max_batch=64, dtype=np.float32, storage="fp16"):
-    """Batched FDTD judge: coupler_analyze results per design, through the
+    """Batched FDTD check: coupler_analyze results per design, through the
     pipelined batched sweep (groups of ``max_batch``, the first of 16: one
     group is built and analysed on the host while the previous one steps on
     the GPU)."""
@@ -91,7 +91,7 @@ def fdtd(designs, steps=W.COUPLER_STEPS, max_batch=64, dtype=np.float32, storage
     res = ParameterSweep.from_samples(designs, batched=runner).run()
     bad = [r for r in res if r["status"] != "success"]
     if bad:
-        raise RuntimeError(f"FDTD judge: {len(bad)} designs failed, first: {bad[0]['error']}")
+        raise RuntimeError(f"FDTD check: {len(bad)} designs failed, first: {bad[0]['error']}")
     return res
 
 
@@ -146,21 +146,21 @@ def run(n_train=4096, n_screen=65536, top_k=64, tau=0.05, members=4, epochs=150,
         f"({top_k} best + {len(pick) - top_k} uncertain)")
 
     t3 = time.perf_counter()
-    judged = fdtd([cand[i] for i in pick], steps=steps)
+    checked = fdtd([cand[i] for i in pick], steps=steps)
     torch.cuda.synchronize()
-    timings["judge_s"] = time.perf_counter() - t3
-    s21f = np.array([r["s21"] for r in judged])
-    s31f = np.array([r["s31"] for r in judged])
+    timings["fdtd_check_s"] = time.perf_counter() - t3
+    s21f = np.array([r["s21"] for r in checked])
+    s31f = np.array([r["s31"] for r in checked])
     ok = obj.accept(s21f, s31f)
     err21 = np.abs(s21p[pick] - s21f)
     err31 = np.abs(s31p[pick] - s31f)
     total = time.perf_counter() - t0
 
     # the method without the surrogate: FDTD on the candidates in their
-    # seeded order, for the same judge budget
+    # seeded order, for the same FDTD budget
     rand_ok = obj.accept(np.array([r["s21"] for r in res]), np.array([r["s31"] for r in res]))
     return {
-        "n_train": n_train, "n_screen": n_screen, "judged": len(pick),
+        "n_train": n_train, "n_screen": n_screen, "fdtd_checked": len(pick),
         "accepted": int(ok.sum()),
         "accept_rate_surrogate_picks": float(ok.mean()),
         "accept_rate_random_designs": float(rand_ok.mean()),

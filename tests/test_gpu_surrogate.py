@@ -2,10 +2,10 @@
 
 The search for 3 dB couplers at 193 THz is run at reduced scale (768
 training spectra from the batched FDTD, 4000 steps, 8192 screened designs,
-32 judged).  Without the surrogate a candidate is accepted at the rate of
-random designs; with it, the designs the FDTD judge confirms among the
+32 checked by FDTD).  Without the surrogate a candidate is accepted at the rate of
+random designs; with it, the designs the FDTD runs confirm among the
 surrogate's picks must be at least 3x as frequent, and the network's
-|S21| at the picks must be within 0.15 of the judge's on average.
+|S21| at the picks must be within 0.15 of the FDTD value on average.
 """
 
 import numpy as np
