@@ -142,7 +142,7 @@ def run(n_train=4096, n_screen=65536, top_k=64, tau=0.05, members=4, epochs=150,
     pick = list(order[:top_k])
     near = np.nonzero((score < 3 * obj.band) & (unc > tau))[0]
     pick += [i for i in near[np.argsort(score[near])] if i not in set(pick)][:top_k]
-    log(f"[surrogate] screened {n_screen} in {timings['screen_s']:.2f} s; judging {len(pick)} "
+    log(f"[surrogate] screened {n_screen} in {timings['screen_s']:.2f} s; checking {len(pick)} by FDTD "
         f"({top_k} best + {len(pick) - top_k} uncertain)")
 
     t3 = time.perf_counter()
