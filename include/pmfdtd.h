@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define PM_ABI_VERSION 5
+#define PM_ABI_VERSION 6
 #define PM_SCRATCH_BYTES 4096
 
 /* error codes (< 0) */

@@ -13,7 +13,7 @@ import os
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libpmfdtd.so"
-ABI_VERSION = 5
+ABI_VERSION = 6
 SCRATCH_BYTES = 4096
 
 PM_EX, PM_EY, PM_EZ, PM_HX, PM_HY, PM_HZ = range(6)
