@@ -3,12 +3,16 @@
 // the weighted MSE loss and gradient, and Adam.
 //
 // GEMM tiling: a workgroup of 4 waves owns a 64 x 64 block of C; each wave a
-// 32 x 32 quarter as 2 x 2 V_MFMA_F32_16X16X32_BF16 tiles.  Per K-step of 32
-// the workgroup stages A (64 x 32) and B (32 x 64) in LDS as BF16, both
-// k-contiguous per row/column, so each lane reads its 8-element operand
-// with one 16-byte LDS load.  The global loads walk the unit-stride side of
-// each operand, whichever it is (forward: X rows and W rows; input gradient:
-// W columns; weight gradient: dY and X columns over the batch).
+// 32 x 32 quarter as 2 x 2 V_MFMA_F32_16X16X32_BF16 tiles.  K goes in blocks
+// of 128: the workgroup stages A (64 x 128) and B (128 x 64) in LDS as BF16,
+// both k-contiguous per row/column, so each lane reads its 8-element operand
+// with one 16-byte LDS load, and runs 4 MFMA K-steps of 32 on the block.
+// The next block's global loads are issued into registers before those
+// MFMAs, so a GEMM of K <= 256 (every surrogate layer) waits for memory
+// about once instead of once per K-step.  The global loads walk the
+// unit-stride side of each operand, whichever it is (forward: X rows and W
+// rows; input gradient: W columns; weight gradient: dY and X columns over
+// the batch).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -24,7 +28,8 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kBM = 64, kBN = 64, kBK = 32, kPad = 8;
+constexpr int kBM = 64, kBN = 64, kBK = 128, kPad = 8;
+constexpr int kStage = kBM * kBK / 256;  // elements of A (and of B) per thread per block
 
 // FP32 -> BF16, round to nearest even (NaN kept quiet)
 __device__ __forceinline__ unsigned short to_bf16(float x) {
@@ -67,55 +72,76 @@ __global__ __launch_bounds__(256) void k_gemm(int mode, int M, int N, int K,
 
   const bool a_kfast = lak == 1;
   const bool b_kfast = lbk == 1;
-  for (int k0 = 0; k0 < K; k0 += kBK) {
-    // stage A: 64 x 32 elements, 8 per thread along the unit-stride axis
+  // element e of this thread's share of a block: (row, k) of A and (n, k) of
+  // B, along the unit-stride axis of each
+  auto a_rc = [&](int e, int& r, int& c) {
+    const int f = tid + 256 * e;
+    if (a_kfast) {
+      r = f / kBK;
+      c = f % kBK;
+    } else {
+      r = f % kBM;
+      c = f / kBM;
+    }
+  };
+  auto b_rc = [&](int e, int& r, int& c) {
+    const int f = tid + 256 * e;
+    if (b_kfast) {
+      r = f / kBK;
+      c = f % kBK;
+    } else {
+      r = f % kBN;
+      c = f / kBN;
+    }
+  };
+  float ra[kStage], rb[kStage];
+  auto gload = [&](int k0) {
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int f = tid + 256 * e;
+    for (int e = 0; e < kStage; ++e) {
       int r, c;
-      if (a_kfast) {
-        r = f / kBK;
-        c = f % kBK;
-      } else {
-        r = f % kBM;
-        c = f / kBM;
-      }
+      a_rc(e, r, c);
       const int m = m0 + r, k = k0 + c;
-      const float v = (m < M && k < K) ? A[(int64_t)m * lam + (int64_t)k * lak] : 0.0f;
-      sa[r][c] = to_bf16(v);
+      ra[e] = (m < M && k < K) ? A[(int64_t)m * lam + (int64_t)k * lak] : 0.0f;
     }
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int f = tid + 256 * e;
-      int r, c;  // r: n within the tile, c: k within the step
-      if (b_kfast) {
-        r = f / kBK;
-        c = f % kBK;
-      } else {
-        r = f % kBN;
-        c = f / kBN;
-      }
+    for (int e = 0; e < kStage; ++e) {
+      int r, c;
+      b_rc(e, r, c);
       const int n = n0 + r, k = k0 + c;
-      const float v = (n < N && k < K) ? B[(int64_t)k * lbk + (int64_t)n * lbn] : 0.0f;
-      sb[r][c] = to_bf16(v);
+      rb[e] = (n < N && k < K) ? B[(int64_t)k * lbk + (int64_t)n * lbn] : 0.0f;
+    }
+  };
+  gload(0);
+  for (int k0 = 0; k0 < K; k0 += kBK) {
+#pragma unroll
+    for (int e = 0; e < kStage; ++e) {
+      int r, c;
+      a_rc(e, r, c);
+      sa[r][c] = to_bf16(ra[e]);
+      b_rc(e, r, c);
+      sb[r][c] = to_bf16(rb[e]);
     }
     __syncthreads();
-    bf16x8 av[2], bv[2];
+    if (k0 + kBK < K) gload(k0 + kBK);  // in flight during this block's MFMAs
 #pragma unroll
-    for (int a = 0; a < 2; ++a) {
-      const unsigned short* p = &sa[wm + 16 * a + li][8 * lq];
-      av[a] = *(const bf16x8*)p;
+    for (int kk = 0; kk < kBK; kk += 32) {
+      bf16x8 av[2], bv[2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const unsigned short* p = &sa[wm + 16 * a + li][kk + 8 * lq];
+        av[a] = *(const bf16x8*)p;
+      }
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const unsigned short* p = &sb[wn + 16 * b + li][kk + 8 * lq];
+        bv[b] = *(const bf16x8*)p;
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a], bv[b], acc[a][b], 0, 0, 0);
     }
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      const unsigned short* p = &sb[wn + 16 * b + li][8 * lq];
-      bv[b] = *(const bf16x8*)p;
-    }
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-      for (int b = 0; b < 2; ++b)
-        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a], bv[b], acc[a][b], 0, 0, 0);
     __syncthreads();
   }
   // epilogue: lane holds rows 4 lq + r, column li of each 16 x 16 tile

@@ -33,17 +33,34 @@ class SpectralMLP:
         self.device = torch.device("cuda", device)
         self.sizes = [n_in] + [hidden] * depth + [n_out]
         rng = np.random.default_rng(seed)
-        self.W, self.b = [], []
-        for k in range(len(self.sizes) - 1):
-            fi, fo = self.sizes[k], self.sizes[k + 1]
-            std = np.sqrt(2.0 / fi) if k < len(self.sizes) - 2 else np.sqrt(1.0 / fi)
-            self.W.append(torch.from_numpy(rng.normal(0, std, (fo, fi)).astype(np.float32)).to(self.device))
-            self.b.append(torch.zeros(fo, dtype=torch.float32, device=self.device))
+        # every weight and bias is a view of one flat buffer (all weights,
+        # then all biases), the gradients of another of the same layout, so
+        # one Adam launch updates everything (two with weight decay)
+        shapes = [(self.sizes[k + 1], self.sizes[k]) for k in range(len(self.sizes) - 1)]
+        self._nW = sum(fo * fi for fo, fi in shapes)
+        total = self._nW + sum(fo for fo, _ in shapes)
+        self._flat = torch.zeros(total, dtype=torch.float32, device=self.device)
+        self._gflat = torch.zeros(total, dtype=torch.float32, device=self.device)
+        self.W, self.b, self._gW, self._gb = [], [], [], []
+        ow, ob = 0, self._nW
+        for k, (fo, fi) in enumerate(shapes):
+            std = np.sqrt(2.0 / fi) if k < len(shapes) - 1 else np.sqrt(1.0 / fi)
+            w = self._flat[ow:ow + fo * fi].view(fo, fi)
+            w.copy_(torch.from_numpy(rng.normal(0, std, (fo, fi)).astype(np.float32)))
+            self.W.append(w)
+            self.b.append(self._flat[ob:ob + fo])
+            self._gW.append(self._gflat[ow:ow + fo * fi].view(fo, fi))
+            self._gb.append(self._gflat[ob:ob + fo])
+            ow += fo * fi
+            ob += fo
         self._adam = None
         self.t = 0
+        self._fixed_stream = None  # set during fit: the stream looked up once
 
     # ---------------------------------------------------------------- helpers
     def _stream(self):
+        if self._fixed_stream is not None:
+            return self._fixed_stream
         return _torch().cuda.current_stream(self.device).cuda_stream
 
     def _check(self, rc, what):
@@ -87,7 +104,8 @@ class SpectralMLP:
         return (a, (acts, zs)) if keep else a
 
     def backward(self, cache, dY):
-        """Gradients [(dW, db)] of sum(dY * Y)."""
+        """Gradients [(dW, db)] of sum(dY * Y), as views of the network's
+        gradient buffer (overwritten by the next call)."""
         torch = _torch()
         acts, zs = cache
         g = dY.contiguous()
@@ -95,10 +113,10 @@ class SpectralMLP:
         grads = [None] * len(self.W)
         for k in range(len(self.W) - 1, -1, -1):
             fo, fi = self.W[k].shape
-            dW = torch.empty((fo, fi), dtype=torch.float32, device=self.device)
+            dW = self._gW[k]
             # dW[o][i] = sum_m g[m][o] a[m][i]: A[m'=o][k'=m] = g[m][o]
             self._gemm(N.MLP_PLAIN, fo, fi, M, g, 1, fo, acts[k], fi, 1, dW, fi)
-            db = torch.empty(fo, dtype=torch.float32, device=self.device)
+            db = self._gb[k]
             self._check(self.lib.pm_mlp_colsum(self._stream(), M, fo, g.data_ptr(), fo,
                                                db.data_ptr()), "pm_mlp_colsum")
             grads[k] = (dW, db)
@@ -123,23 +141,50 @@ class SpectralMLP:
     def adam_step(self, grads, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8, wd=0.0):
         torch = _torch()
         if self._adam is None:
-            self._adam = [(torch.zeros_like(w), torch.zeros_like(w), torch.zeros_like(b),
-                           torch.zeros_like(b)) for w, b in zip(self.W, self.b)]
+            self._adam = (torch.zeros_like(self._flat), torch.zeros_like(self._flat))
+        mf, vf = self._adam
         self.t += 1
-        for (W, b), (gW, gb), (mW, vW, mb, vb) in zip(zip(self.W, self.b), grads, self._adam):
-            for p, g, m, v, decay in ((W, gW, mW, vW, wd), (b, gb, mb, vb, 0.0)):
-                self._check(self.lib.pm_mlp_adam(self._stream(), p.numel(), p.data_ptr(),
+        flat = all(gW.data_ptr() == a.data_ptr() and gb.data_ptr() == c.data_ptr()
+                   for (gW, gb), a, c in zip(grads, self._gW, self._gb))
+        if flat:
+            # the whole buffer at once (weights then biases; decay on weights)
+            spans = [(0, self._flat.numel(), wd)] if wd == 0.0 else \
+                [(0, self._nW, wd), (self._nW, self._flat.numel(), 0.0)]
+            for a, e, decay in spans:
+                p, g, m, v = (t[a:e] for t in (self._flat, self._gflat, mf, vf))
+                self._check(self.lib.pm_mlp_adam(self._stream(), e - a, p.data_ptr(),
                                                  g.data_ptr(), m.data_ptr(), v.data_ptr(),
                                                  lr, b1, b2, eps, decay, self.t), "pm_mlp_adam")
+            return
+        # gradients given elsewhere: one launch per tensor, moments from the
+        # same flat buffers
+        ow, ob = 0, self._nW
+        for (W, b), (gW, gb) in zip(zip(self.W, self.b), grads):
+            for p, g, o, decay in ((W, gW, ow, wd), (b, gb, ob, 0.0)):
+                n = p.numel()
+                self._check(self.lib.pm_mlp_adam(self._stream(), n, p.data_ptr(),
+                                                 g.contiguous().data_ptr(),
+                                                 mf[o:o + n].data_ptr(), vf[o:o + n].data_ptr(),
+                                                 lr, b1, b2, eps, decay, self.t), "pm_mlp_adam")
+            ow += W.numel()
+            ob += b.numel()
 
     def fit(self, X, T, epochs: int = 200, batch: int = 256, lr: float = 2e-3, w=None,
             seed: int = 0, lr_final: float = 1e-4, wd: float = 0.0):
         """Minibatch Adam with a cosine learning-rate decay; returns the
         per-epoch mean loss (host floats)."""
-        torch = _torch()
-        M = X.shape[0]
         rng = np.random.default_rng(seed)
         hist = []
+        self._fixed_stream = self._stream()
+        try:
+            self._fit_epochs(X, T, epochs, batch, lr, w, rng, lr_final, wd, hist)
+        finally:
+            self._fixed_stream = None
+        return hist
+
+    def _fit_epochs(self, X, T, epochs, batch, lr, w, rng, lr_final, wd, hist):
+        torch = _torch()
+        M = X.shape[0]
         steps = epochs * ((M + batch - 1) // batch)
         s = 0
         for ep in range(epochs):
@@ -158,7 +203,6 @@ class SpectralMLP:
                 nb += 1
                 s += 1
             hist.append(float(tot.item()) / nb)
-        return hist
 
 
 class SurrogateEnsemble:

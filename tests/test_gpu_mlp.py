@@ -27,7 +27,8 @@ def _rel(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
 
 
-@pytest.mark.parametrize("M,Nn,K", [(70, 45, 37), (128, 256, 256), (5, 300, 4), (257, 64, 96)])
+@pytest.mark.parametrize("M,Nn,K", [(70, 45, 37), (128, 256, 256), (5, 300, 4), (257, 64, 96),
+                                    (33, 70, 300)])
 def test_gemm_modes_match_oracle(mi355x, M, Nn, K):
     import torch
     from oracle import mlp as O2
