@@ -210,7 +210,8 @@ template <typename T, int CK, int VEC, int W, int TY, bool YZ, bool ADE>
 __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kchunk,
                                            int ntile_y, int nchunk, int nblocks, int order,
                                            const TileBox& bx, int b,
-                                           FusedLds<T, VEC, W, TY>& lds, const KAde* tab) {
+                                           FusedLds<T, VEC, W, TY>& lds, const KAde* tab,
+                                           const T (*tcf)[12]) {
   constexpr int RW = 64 * VEC * W;
   using ABits = std::conditional_t<(12 * VEC > 32), uint64_t, uint32_t>;
   static_assert(kMaxAde < 16, "box numbers are kept in 4 bits");
@@ -622,15 +623,18 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
             const int tz = (int)((abits >> (4 * (2 * VEC + e))) & 15) - 1;
             if (exok && tx >= 0) {
               const KAde& a = tab[tx];
-              vx = ade_cell_state<T>(a, ade_index(a, i, j, kk), ap[0][e], aq[0][e], cx, exm[e]);
+              vx = ade_cell_coef<T>(a, tcf[tx], ade_index(a, i, j, kk), ap[0][e], aq[0][e], cx,
+                                    exm[e]);
             }
             if (eyok && ty_ >= 0) {
               const KAde& a = tab[ty_];
-              vy = ade_cell_state<T>(a, ade_index(a, i, j, kk), ap[1][e], aq[1][e], cy, eym[e]);
+              vy = ade_cell_coef<T>(a, tcf[ty_], ade_index(a, i, j, kk), ap[1][e], aq[1][e], cy,
+                                    eym[e]);
             }
             if (ezok && tz >= 0) {
               const KAde& a = tab[tz];
-              vz = ade_cell_state<T>(a, ade_index(a, i, j, kk), ap[2][e], aq[2][e], cz, ezm[e]);
+              vz = ade_cell_coef<T>(a, tcf[tz], ade_index(a, i, j, kk), ap[2][e], aq[2][e], cz,
+                                    ezm[e]);
             }
           }
         }
@@ -716,6 +720,9 @@ __global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, 
   // so the per-row scans read LDS instead of issuing global loads whose
   // waits would also wait for the plane prefetch in flight
   __shared__ KAde ade_tab[ADE ? kMaxAde : 1];
+  // and the nine update coefficients of each box in the field type (the
+  // E update then reads them without a conversion)
+  __shared__ T ade_cf[ADE ? kMaxAde : 1][12];
   if constexpr (ADE) {
     static_assert(sizeof(KAde) % 4 == 0, "KAde is copied as 32-bit words");
     const int n = g.n_ade * (int)(sizeof(KAde) / 4);
@@ -723,14 +730,22 @@ __global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, 
     uint32_t* dst = (uint32_t*)ade_tab;
     for (int t = threadIdx.x; t < n; t += blockDim.x) dst[t] = src[t];
     __syncthreads();
+    if ((int)threadIdx.x < g.n_ade * 9) {
+      const int bq = threadIdx.x / 9, q = threadIdx.x % 9;
+      const KAde& a = ade_tab[bq];
+      const double v = q == 0 ? a.ke : q == 1 ? a.kp : q == 2 ? a.kq : q == 3 ? a.ks
+                     : q == 4 ? a.a0 : q == 5 ? a.a1 : q == 6 ? a.a2 : q == 7 ? a.a3 : a.a4;
+      ade_cf[bq][q] = (T)v;
+    }
+    __syncthreads();
   }
   const int b = blockIdx.x;
   if (b < seg)
     fused_tile<T, CK, VEC, W, TY, true, ADE>(g, z, kchunk, ntile_y, nchunk, n_out, order, bx,
-                                             b, lds, ade_tab);
+                                             b, lds, ade_tab, ade_cf);
   else
     fused_tile<T, CK, VEC, W, TY, false, ADE>(g, z, kchunk, ntile_y, nchunk, n_in, order, bx,
-                                              b - seg, lds, ade_tab);
+                                              b - seg, lds, ade_tab, ade_cf);
 }
 
 template <typename T, int VEC, int W, int TY>

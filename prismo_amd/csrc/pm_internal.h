@@ -109,6 +109,24 @@ __device__ __forceinline__ T ade_cell_state(const KAde& a, int64_t l, T p0, T q0
   return en;
 }
 
+// The same with the nine coefficients already in the field type, cf = ke
+// kp kq ks a0 a1 a2 a3 a4 (the values (T)a.ke ... (T)a.a4).
+template <typename T>
+__device__ __forceinline__ T ade_cell_coef(const KAde& a, const T* cf, int64_t l, T p0, T q0,
+                                           T curl, T e) {
+  const gptr<T> P = (gptr<T>)a.p;
+  if (a.kind == 1) {
+    const gptr<T> Q = (gptr<T>)a.q;
+    const T en = cf[0] * e + cf[1] * p0 - cf[2] * q0 + cf[3] * curl;
+    P[l] = cf[4] * en + cf[5] * e + cf[6] * p0 + q0;
+    Q[l] = cf[7] * p0 + cf[8] * e;
+    return en;
+  }
+  const T en = cf[0] * e + cf[1] * p0 + cf[3] * curl;
+  P[l] = cf[4] * en + cf[5] * e + cf[6] * p0;
+  return en;
+}
+
 template <typename T>
 __device__ __forceinline__ T ade_cell(const KAde& a, int64_t l, T curl, T e) {
   const T p0 = ((gptr<const T>)a.p)[l];
