@@ -391,6 +391,32 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
     fld(eym, zp<T>(z.bi, z.cs, o));
   }
 
+  // dispersive boxes this wave's row cuts, per component (uniform), and per
+  // lane those whose x range holds cell e: the z-march changes only the
+  // plane, so each plane then needs the boxes' plane ranges alone
+  uint32_t rc0 = 0, rc1 = 0, rc2 = 0;
+  uint32_t xm[VEC];
+#pragma unroll
+  for (int e = 0; e < VEC; ++e) xm[e] = 0;
+  if constexpr (ADE) {
+    if (row_ok && j >= g.ade_j0 && j < g.ade_j1) {
+      for (int t = 0; t < g.n_ade; ++t) {
+        const KAde& a = tab[t];
+        if (j < a.j0 || j >= a.j1) continue;
+        const uint32_t bit = 1u << t;
+        if (a.comp == 0)
+          rc0 |= bit;
+        else if (a.comp == 1)
+          rc1 |= bit;
+        else
+          rc2 |= bit;
+#pragma unroll
+        for (int e = 0; e < VEC; ++e)
+          if (x0 + e >= a.i0 && x0 + e < a.i1) xm[e] |= bit;
+      }
+    }
+  }
+
   auto step = [&](const int k, const PlaneIn& cur, PlaneIn& nxt) __attribute__((always_inline)) {
     const uint32_t off = rowoff + (uint32_t)k * sz;
     const bool do_e = k > kb && own;
@@ -415,14 +441,15 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
         ze[t] = *zp<T>(z.bc, z.prof[2] + (uint32_t)t * z.pn[2], kq);
       }
     }
-    // (1b) dispersive rows: the box of each lane's cell per component (boxes
-    //      scanned in order, the first one holding the cell takes it) from
-    //      the LDS table, then its state (P, and Q for Lorentz), issued with
-    //      the E-side loads so the E update does not wait for them.  No loop
-    //      here or in (6) issues a memory access: with a loop-dependent
-    //      count of accesses in flight the compiler would wait for all of
-    //      them, the plane prefetch included.
-    const bool ade_row = ADE && do_e && row_has_ade(g, tab, j, kk);
+    // (1b) dispersive rows: the box of each lane's cell per component (the
+    //      first box holding the cell takes it: the lowest set bit of the
+    //      row's, the plane's and the lane's box masks), then its state (P,
+    //      and Q for Lorentz), issued with the E-side loads so the E update
+    //      does not wait for them.  No loop here or in (6) issues a memory
+    //      access: with a loop-dependent count of accesses in flight the
+    //      compiler would wait for all of them, the plane prefetch included.
+    const bool ade_row =
+        ADE && do_e && (rc0 | rc1 | rc2) != 0 && kk >= g.ade_k0 && kk < g.ade_k1;
     // box number + 1 of (component c, cell e) in 4 bits at 4 (c VEC + e);
     // 0: no box
     ABits abits = 0;
@@ -433,19 +460,18 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
 #pragma unroll
         for (int e = 0; e < VEC; ++e) ap[c][e] = aq[c][e] = (T)0;
       if (ade_row) {
+        uint32_t km = 0;  // boxes whose plane range holds kk
         for (int t = 0; t < g.n_ade; ++t) {
           const KAde& a = tab[t];
-          if (j < a.j0 || j >= a.j1 || kk < a.k0 || kk >= a.k1) continue;
+          if (kk >= a.k0 && kk < a.k1) km |= 1u << t;
+        }
 #pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            if (a.comp != c) continue;
+        for (int c = 0; c < 3; ++c) {
+          const uint32_t cm = (c == 0 ? rc0 : c == 1 ? rc1 : rc2) & km;
 #pragma unroll
-            for (int e = 0; e < VEC; ++e) {
-              const int i = x0 + e;
-              const int sh = 4 * (c * VEC + e);
-              if (((abits >> sh) & 15) == 0 && i >= a.i0 && i < a.i1)
-                abits |= (ABits)(t + 1) << sh;
-            }
+          for (int e = 0; e < VEC; ++e) {
+            const uint32_t m = cm & xm[e];
+            if (m) abits |= (ABits)(__builtin_ctz(m) + 1) << (4 * (c * VEC + e));
           }
         }
         // every lane loads: one without a box reads the first word of the

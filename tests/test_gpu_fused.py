@@ -158,6 +158,31 @@ def test_fused_ade_lorentz_drude(mi355x):
     assert rel_l2(_all(_fields(sim)), _all(o1.F)) <= 1e-10
 
 
+@pytest.mark.parametrize("ty,kc,vec,order", [(2, 8, 1, 2), (3, 5, 1, 3), (4, 16, 2, 2),
+                                             (2, 3, 2, 3)])
+def test_fused_ade_fp32_tilings(mi355x, ty, kc, vec, order):
+    """FP32 fields with Lorentz and Drude boxes that cross tile rows and
+    z-chunks, under several launch shapes, against the FP64 oracle fed the
+    same coefficients: the difference is FP32 arithmetic only."""
+    import prismo_amd as pa
+
+    sim, eps = _sim(100, 24, 22, t=4, dtype=np.float32)
+    si = pa.LorentzMaterial(7.98, [pa.LorentzPole(omega_0=1.1e16, delta_epsilon=3.69, gamma=1e13)])
+    au = pa.DrudeMaterial(1.0, 1.37e16, 1.07e14)
+    boxes = sim.add_dispersive_box(si, (4 * D, 2 * D, 2 * D), (12 * D, 9 * D, 8 * D))
+    boxes += sim.add_dispersive_box(au, (30 * D, 0.0, 0.0), (40 * D, 10 * D, 10 * D))
+    eng = sim.solver.updater.engine_for(sim.fields)
+    eng.ctx.set_tuning(True, ty, kc, vec, order)
+    assert eng.ctx.fused_active() and len(boxes) == 6
+    steps = 60
+    sim.run(steps * sim.dt * 0.9999)
+    o1 = _oracle(sim, eps, 4)
+    for reg in sim._ade:
+        o1.add_ade(reg.comp, reg.box, reg.kind, reg.eps_inf, reg.c)
+    _drive(o1, sim, steps)
+    assert rel_l2(_all(_fields(sim)), _all(o1.F)) <= 1e-5
+
+
 def test_autotune_restores_state(mi355x):
     from prismo_amd import autotune
 
