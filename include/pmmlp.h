@@ -45,6 +45,22 @@ int pm_mlp_colsum(void* stream, int32_t M, int32_t N, const float* A, int64_t ld
 int pm_mlp_mse(void* stream, int32_t M, int32_t N, const float* Y, const float* T,
                const float* w, float* dY, float* loss);
 
+/* Batched forms: `batch` independent problems of the same shape (1 <=
+ * batch <= 65535), problem q reading and writing its operands at q times
+ * the given stride (in elements) from the base pointers; each problem's
+ * result is the one the single call gives.  Loss of problem q in loss[q].
+ * No counterpart in the reference: the surrogate trains its ensemble
+ * members in lockstep, one launch for all members. */
+int pm_mlp_gemm_batched(void* stream, int mode, int32_t batch, int32_t M, int32_t N, int32_t K,
+                        const float* A, int64_t lda_m, int64_t lda_k, int64_t stride_a,
+                        const float* B, int64_t ldb_k, int64_t ldb_n, int64_t stride_b,
+                        float* C, int64_t ldc, int64_t stride_c, const float* bias,
+                        int64_t stride_bias, float* Z, int64_t stride_z);
+int pm_mlp_colsum_batched(void* stream, int32_t batch, int32_t M, int32_t N, const float* A,
+                          int64_t lda, int64_t stride_a, float* out, int64_t stride_out);
+int pm_mlp_mse_batched(void* stream, int32_t batch, int32_t M, int32_t N, const float* Y,
+                       const float* T, int64_t stride, const float* w, float* dY, float* loss);
+
 /* Adam with decoupled weight decay on n FP32 parameters, step t >= 1. */
 int pm_mlp_adam(void* stream, int64_t n, float* p, const float* g, float* m, float* v,
                 float lr, float beta1, float beta2, float eps, float weight_decay, int32_t t);

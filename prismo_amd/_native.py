@@ -155,6 +155,10 @@ MLP_SIGNATURES = {
     "pm_mlp_mse": ([vp, i32, i32, vp, vp, vp, vp, vp], C.c_int),
     "pm_mlp_adam": ([vp, i64, vp, vp, vp, vp, C.c_float, C.c_float, C.c_float, C.c_float,
                      C.c_float, i32], C.c_int),
+    "pm_mlp_gemm_batched": ([vp, C.c_int, i32, i32, i32, i32, vp, i64, i64, i64, vp, i64, i64,
+                             i64, vp, i64, i64, vp, i64, vp, i64], C.c_int),
+    "pm_mlp_colsum_batched": ([vp, i32, i32, i32, vp, i64, i64, vp, i64], C.c_int),
+    "pm_mlp_mse_batched": ([vp, i32, i32, i32, vp, vp, i64, vp, vp, vp], C.c_int),
 }
 
 BATCH_SIGNATURES = {

@@ -50,12 +50,25 @@ __device__ __forceinline__ float dgelu(float z) {
   return cdf + z * pdf;
 }
 
+// Operand strides per problem of a batch (blockIdx.z), 0 for a single GEMM.
+struct GemmBatch {
+  int64_t a, b, c, bias, z;
+};
+
 __global__ __launch_bounds__(256) void k_gemm(int mode, int M, int N, int K,
                                               const float* __restrict__ A, int64_t lam,
                                               int64_t lak, const float* __restrict__ B,
                                               int64_t lbk, int64_t lbn, float* __restrict__ C,
                                               int64_t ldc, const float* __restrict__ bias,
-                                              float* __restrict__ Z) {
+                                              float* __restrict__ Z, GemmBatch bs) {
+  {
+    const int64_t q = blockIdx.z;
+    A += q * bs.a;
+    B += q * bs.b;
+    C += q * bs.c;
+    if (bias) bias += q * bs.bias;
+    if (Z) Z += q * bs.z;
+  }
   __shared__ unsigned short sa[kBM][kBK + kPad];
   __shared__ unsigned short sb[kBN][kBK + kPad];
   const int tid = threadIdx.x;
@@ -172,8 +185,11 @@ __global__ __launch_bounds__(256) void k_gemm(int mode, int M, int N, int K,
 
 // one workgroup per 64 columns; 4 waves split the rows, then combine in LDS
 __global__ __launch_bounds__(256) void k_colsum(int M, int N, const float* __restrict__ A,
-                                                int64_t lda, float* __restrict__ out) {
+                                                int64_t lda, float* __restrict__ out,
+                                                int64_t sa, int64_t so) {
   __shared__ float part[4][64];
+  A += (int64_t)blockIdx.y * sa;  // problem of a batch
+  out += (int64_t)blockIdx.y * so;
   const int c = blockIdx.x * 64 + (threadIdx.x & 63);
   const int w = threadIdx.x >> 6;
   float s = 0.0f;
@@ -189,8 +205,15 @@ __global__ __launch_bounds__(256) void k_colsum(int M, int N, const float* __res
 __global__ __launch_bounds__(256) void k_mse(int M, int N, const float* __restrict__ Y,
                                              const float* __restrict__ T,
                                              const float* __restrict__ w,
-                                             float* __restrict__ dY, float* loss) {
+                                             float* __restrict__ dY, float* loss, int64_t sy) {
   __shared__ float red[256];
+  {
+    const int64_t q = blockIdx.y;  // problem of a batch: Y, T, dY at q sy, loss[q]
+    Y += q * sy;
+    T += q * sy;
+    dY += q * sy;
+    loss += q;
+  }
   const int64_t total = (int64_t)M * N;
   const float scale = 1.0f / (float)total;
   float s = 0.0f;
@@ -243,31 +266,53 @@ extern "C" {
 int pm_mlp_gemm(void* stream, int mode, int32_t M, int32_t N, int32_t K, const float* A,
                 int64_t lda_m, int64_t lda_k, const float* B, int64_t ldb_k, int64_t ldb_n,
                 float* C, int64_t ldc, const float* bias, float* Z) {
+  return pm_mlp_gemm_batched(stream, mode, 1, M, N, K, A, lda_m, lda_k, 0, B, ldb_k, ldb_n, 0,
+                             C, ldc, 0, bias, 0, Z, 0);
+}
+
+int pm_mlp_gemm_batched(void* stream, int mode, int32_t batch, int32_t M, int32_t N, int32_t K,
+                        const float* A, int64_t lda_m, int64_t lda_k, int64_t stride_a,
+                        const float* B, int64_t ldb_k, int64_t ldb_n, int64_t stride_b,
+                        float* C, int64_t ldc, int64_t stride_c, const float* bias,
+                        int64_t stride_bias, float* Z, int64_t stride_z) {
   if (M < 0 || N < 0 || K < 0 || mode < 0 || mode > 3 || !A || !B || !C) return PM_EINVAL;
+  if (batch < 1 || batch > 65535) return PM_EINVAL;
   if ((mode == PM_MLP_GELU || mode == PM_MLP_DGELU) && !Z) return PM_EINVAL;
   if (M == 0 || N == 0) return 0;
-  dim3 grid((N + kBN - 1) / kBN, (M + kBM - 1) / kBM);
+  dim3 grid((N + kBN - 1) / kBN, (M + kBM - 1) / kBM, batch);
   if (grid.y > 65535) return PM_EINVAL;
+  const GemmBatch bs{stride_a, stride_b, stride_c, stride_bias, stride_z};
   hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, (hipStream_t)stream, mode, M, N, K, A,
-                     lda_m, lda_k, B, ldb_k, ldb_n, C, ldc, bias, Z);
+                     lda_m, lda_k, B, ldb_k, ldb_n, C, ldc, bias, Z, bs);
   return status();
 }
 
 int pm_mlp_colsum(void* stream, int32_t M, int32_t N, const float* A, int64_t lda, float* out) {
-  if (M < 0 || N < 0 || !A || !out) return PM_EINVAL;
+  return pm_mlp_colsum_batched(stream, 1, M, N, A, lda, 0, out, 0);
+}
+
+int pm_mlp_colsum_batched(void* stream, int32_t batch, int32_t M, int32_t N, const float* A,
+                          int64_t lda, int64_t stride_a, float* out, int64_t stride_out) {
+  if (M < 0 || N < 0 || !A || !out || batch < 1 || batch > 65535) return PM_EINVAL;
   if (N == 0) return 0;
-  hipLaunchKernelGGL(k_colsum, dim3((N + 63) / 64), dim3(256), 0, (hipStream_t)stream, M, N,
-                     A, lda, out);
+  hipLaunchKernelGGL(k_colsum, dim3((N + 63) / 64, batch), dim3(256), 0, (hipStream_t)stream,
+                     M, N, A, lda, out, stride_a, stride_out);
   return status();
 }
 
 int pm_mlp_mse(void* stream, int32_t M, int32_t N, const float* Y, const float* T,
                const float* w, float* dY, float* loss) {
-  if (M <= 0 || N <= 0 || !Y || !T || !dY || !loss) return PM_EINVAL;
+  return pm_mlp_mse_batched(stream, 1, M, N, Y, T, 0, w, dY, loss);
+}
+
+int pm_mlp_mse_batched(void* stream, int32_t batch, int32_t M, int32_t N, const float* Y,
+                       const float* T, int64_t stride, const float* w, float* dY, float* loss) {
+  if (M <= 0 || N <= 0 || !Y || !T || !dY || !loss || batch < 1 || batch > 65535)
+    return PM_EINVAL;
   const int64_t total = (int64_t)M * N;
   const int blocks = (int)std::min<int64_t>((total + 255) / 256, 1024);
-  hipLaunchKernelGGL(k_mse, dim3(blocks), dim3(256), 0, (hipStream_t)stream, M, N, Y, T, w,
-                     dY, loss);
+  hipLaunchKernelGGL(k_mse, dim3(blocks, batch), dim3(256), 0, (hipStream_t)stream, M, N, Y, T,
+                     w, dY, loss, stride);
   return status();
 }
 

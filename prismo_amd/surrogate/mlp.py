@@ -36,26 +36,39 @@ class SpectralMLP:
         # every weight and bias is a view of one flat buffer (all weights,
         # then all biases), the gradients of another of the same layout, so
         # one Adam launch updates everything (two with weight decay)
-        shapes = [(self.sizes[k + 1], self.sizes[k]) for k in range(len(self.sizes) - 1)]
-        self._nW = sum(fo * fi for fo, fi in shapes)
-        total = self._nW + sum(fo for fo, _ in shapes)
-        self._flat = torch.zeros(total, dtype=torch.float32, device=self.device)
-        self._gflat = torch.zeros(total, dtype=torch.float32, device=self.device)
-        self.W, self.b, self._gW, self._gb = [], [], [], []
-        ow, ob = 0, self._nW
-        for k, (fo, fi) in enumerate(shapes):
-            std = np.sqrt(2.0 / fi) if k < len(shapes) - 1 else np.sqrt(1.0 / fi)
-            w = self._flat[ow:ow + fo * fi].view(fo, fi)
-            w.copy_(torch.from_numpy(rng.normal(0, std, (fo, fi)).astype(np.float32)))
-            self.W.append(w)
-            self.b.append(self._flat[ob:ob + fo])
-            self._gW.append(self._gflat[ow:ow + fo * fi].view(fo, fi))
-            self._gb.append(self._gflat[ob:ob + fo])
-            ow += fo * fi
-            ob += fo
+        self.shapes = [(self.sizes[k + 1], self.sizes[k]) for k in range(len(self.sizes) - 1)]
+        self._nW = sum(fo * fi for fo, fi in self.shapes)
+        total = self._nW + sum(fo for fo, _ in self.shapes)
+        self._carve(torch.zeros(total, dtype=torch.float32, device=self.device),
+                    torch.zeros(total, dtype=torch.float32, device=self.device))
+        for k, (fo, fi) in enumerate(self.shapes):
+            std = np.sqrt(2.0 / fi) if k < len(self.shapes) - 1 else np.sqrt(1.0 / fi)
+            self.W[k].copy_(torch.from_numpy(rng.normal(0, std, (fo, fi)).astype(np.float32)))
         self._adam = None
         self.t = 0
         self._fixed_stream = None  # set during fit: the stream looked up once
+
+    def _carve(self, flat, gflat) -> None:
+        """Weights and biases as views of ``flat``, their gradients of
+        ``gflat`` (both 1-D, all weights then all biases)."""
+        self._flat, self._gflat = flat, gflat
+        self.W, self.b, self._gW, self._gb = [], [], [], []
+        ow, ob = 0, self._nW
+        for fo, fi in self.shapes:
+            self.W.append(flat[ow:ow + fo * fi].view(fo, fi))
+            self.b.append(flat[ob:ob + fo])
+            self._gW.append(gflat[ow:ow + fo * fi].view(fo, fi))
+            self._gb.append(gflat[ob:ob + fo])
+            ow += fo * fi
+            ob += fo
+
+    def woff(self, k) -> int:
+        """Offset of layer k's weights in the flat buffer."""
+        return sum(fo * fi for fo, fi in self.shapes[:k])
+
+    def boff(self, k) -> int:
+        """Offset of layer k's biases in the flat buffer."""
+        return self._nW + sum(fo for fo, _ in self.shapes[:k])
 
     # ---------------------------------------------------------------- helpers
     def _stream(self):
@@ -207,13 +220,122 @@ class SpectralMLP:
 
 class SurrogateEnsemble:
     """Members trained from different seeds on the same data; prediction =
-    member mean, uncertainty = member standard deviation."""
+    member mean, uncertainty = member standard deviation.
+
+    The members' parameters live in one (members, P) buffer (each member's
+    ``SpectralMLP`` views its row), and ``fit`` trains them in lockstep:
+    every GEMM, column sum and loss of a step is one batched launch over the
+    members and Adam one launch over the buffer.  Each member sees what its
+    own ``fit(seed=k)`` would (same initial weights, same minibatch order):
+    the parameters come out the same."""
 
     def __init__(self, n_members: int = 4, **kw):
+        torch = _torch()
         self.members = [SpectralMLP(seed=1000 + k, **kw) for k in range(n_members)]
+        m0 = self.members[0]
+        self.P = torch.stack([m._flat for m in self.members]).contiguous()
+        self.G = torch.zeros_like(self.P)
+        for k, m in enumerate(self.members):
+            m._carve(self.P[k], self.G[k])
+        self._adam = None
+        self.t = 0
+        self.lib, self.device = m0.lib, m0.device
 
-    def fit(self, X, T, **kw):
-        return [m.fit(X, T, seed=k, **kw) for k, m in enumerate(self.members)]
+    def _check(self, rc, what):
+        if rc != 0:
+            raise N.NativeError(f"{what} failed ({rc})")
+
+    def fit(self, X, T, epochs: int = 200, batch: int = 256, lr: float = 2e-3, w=None,
+            lr_final: float = 1e-4, wd: float = 0.0):
+        """Minibatch Adam with a cosine learning-rate decay, member k on the
+        minibatch order of seed k; returns each member's per-epoch mean
+        loss (host floats)."""
+        torch = _torch()
+        L, nm = self.members[0], len(self.members)
+        lib, dev = self.lib, self.device
+        st = torch.cuda.current_stream(dev).cuda_stream
+        Pn = self.P.shape[1]
+        M = X.shape[0]
+        rngs = [np.random.default_rng(k) for k in range(nm)]
+        hist = [[] for _ in range(nm)]
+        steps = epochs * ((M + batch - 1) // batch)
+        if self._adam is None:
+            self._adam = (torch.zeros_like(self.P), torch.zeros_like(self.P))
+        mA, vA = self._adam
+        P0, G0 = self.P.data_ptr(), self.G.data_ptr()
+        f4 = 4  # bytes per element
+        s = 0
+        for ep in range(epochs):
+            perms = torch.stack([torch.from_numpy(r.permutation(M)) for r in rngs]).to(dev)
+            tot = torch.zeros(nm, dtype=torch.float32, device=dev)
+            nb = 0
+            for i0 in range(0, M, batch):
+                idx = perms[:, i0:i0 + batch]
+                B = idx.shape[1]
+                a = X[idx].contiguous()  # (nm, B, n_in)
+                tb = T[idx].contiguous()
+                acts, zs = [a], []
+                for k, (fo, fi) in enumerate(L.shapes):
+                    out = torch.empty((nm, B, fo), dtype=torch.float32, device=dev)
+                    last = k == len(L.shapes) - 1
+                    z = None if last else torch.empty_like(out)
+                    self._check(lib.pm_mlp_gemm_batched(
+                        st, N.MLP_LINEAR if last else N.MLP_GELU, nm, B, fo, fi,
+                        a.data_ptr(), fi, 1, B * fi, P0 + f4 * L.woff(k), 1, fi, Pn,
+                        out.data_ptr(), fo, B * fo, P0 + f4 * L.boff(k), Pn,
+                        z.data_ptr() if z is not None else None, B * fo), "gemm")
+                    if z is not None:
+                        zs.append(z)
+                    acts.append(out)
+                    a = out
+                Y = a
+                nout = Y.shape[2]
+                dY = torch.empty_like(Y)
+                loss = torch.zeros(nm, dtype=torch.float32, device=dev)
+                self._check(lib.pm_mlp_mse_batched(
+                    st, nm, B, nout, Y.data_ptr(), tb.data_ptr(), B * nout,
+                    w.data_ptr() if w is not None else None, dY.data_ptr(), loss.data_ptr()),
+                    "mse")
+                g = dY
+                for k in range(len(L.shapes) - 1, -1, -1):
+                    fo, fi = L.shapes[k]
+                    # dW[o][i] = sum_m g[m][o] a[m][i]
+                    self._check(lib.pm_mlp_gemm_batched(
+                        st, N.MLP_PLAIN, nm, fo, fi, B, g.data_ptr(), 1, fo, B * fo,
+                        acts[k].data_ptr(), fi, 1, B * fi, G0 + f4 * L.woff(k), fi, Pn,
+                        None, 0, None, 0), "gemm")
+                    self._check(lib.pm_mlp_colsum_batched(
+                        st, nm, B, fo, g.data_ptr(), fo, B * fo, G0 + f4 * L.boff(k), Pn),
+                        "colsum")
+                    if k > 0:
+                        gp = torch.empty((nm, B, fi), dtype=torch.float32, device=dev)
+                        self._check(lib.pm_mlp_gemm_batched(
+                            st, N.MLP_DGELU, nm, B, fi, fo, g.data_ptr(), fo, 1, B * fo,
+                            P0 + f4 * L.woff(k), fi, 1, Pn, gp.data_ptr(), fi, B * fi,
+                            None, 0, zs[k - 1].data_ptr(), B * fi), "gemm")
+                        g = gp
+                lr_s = lr_final + 0.5 * (lr - lr_final) * (1 + np.cos(np.pi * s / max(1, steps)))
+                self.t += 1
+                if wd == 0.0:
+                    spans = [(0, self.P.numel(), 0.0)]
+                else:
+                    spans = []
+                    for q in range(nm):
+                        spans += [(q * Pn, q * Pn + L._nW, wd), (q * Pn + L._nW, (q + 1) * Pn, 0.0)]
+                for a0, e0, decay in spans:
+                    self._check(lib.pm_mlp_adam(
+                        st, e0 - a0, P0 + f4 * a0, G0 + f4 * a0, mA.data_ptr() + f4 * a0,
+                        vA.data_ptr() + f4 * a0, float(lr_s), 0.9, 0.999, 1e-8, decay, self.t),
+                        "adam")
+                tot += loss
+                nb += 1
+                s += 1
+            h = tot.cpu().numpy()
+            for q in range(nm):
+                hist[q].append(float(h[q]) / nb)
+        for m in self.members:
+            m.t = self.t
+        return hist
 
     def predict(self, X, chunk: int = 16384):
         torch = _torch()

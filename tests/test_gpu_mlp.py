@@ -145,3 +145,24 @@ def test_training_reduces_loss(mi355x):
     net = SpectralMLP(n_in=4, n_out=8, hidden=64, depth=3, seed=2)
     hist = net.fit(_t(X), _t(T), epochs=120, batch=64, lr=3e-3)
     assert hist[-1] < hist[0] / 20, (hist[0], hist[-1])
+
+
+def test_ensemble_lockstep_equals_members_one_by_one(mi355x):
+    """Batched lockstep training of the ensemble (one launch per GEMM,
+    column sum and loss for all members) gives every member the parameters
+    its own fit(seed=k) gives; the loss histories agree to rounding (the loss
+    sum uses atomics)."""
+    from prismo_amd.surrogate.mlp import SpectralMLP, SurrogateEnsemble
+
+    rng = np.random.default_rng(3)
+    X = rng.uniform(-1, 1, (200, 4))
+    T = np.tanh(X @ rng.normal(size=(4, 12)))
+    kw = dict(n_in=4, n_out=12, hidden=48, depth=3)
+    ens = SurrogateEnsemble(n_members=3, **kw)
+    hist = ens.fit(_t(X), _t(T), epochs=4, batch=64, lr=3e-3)
+    for k, m in enumerate(ens.members):
+        ref = SpectralMLP(seed=1000 + k, **kw)
+        h = ref.fit(_t(X), _t(T), epochs=4, batch=64, lr=3e-3, seed=k)
+        for (w1, b1), (w2, b2) in zip(m.params(), ref.params()):
+            assert np.array_equal(w1, w2) and np.array_equal(b1, b2), k
+        assert np.allclose(hist[k], h, rtol=1e-5), (k, hist[k], h)
