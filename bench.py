@@ -35,7 +35,12 @@ steps); small grids are launch-bound, so it reports the microseconds per
 step.  ``waveguide_3d``: the dispersive waveguide (480 x 240 x 160 cells,
 Lorentz core, Drude pad, mode source, 64-frequency DFT plane, 6000 steps) in
 FP32 and FP64 with the relative L2 difference of the plane's spectra.
-``--no-strip`` / ``--no-waveguide`` / ``--no-sweep`` skip those parts.
+``voxel_fp64``: the voxel workload with FP64 fields and coefficients (the
+plain method), the same steps: throughput, its roofline (120 B per cell,
+64 B per CPML cell-axis) and the relative L2 difference over all six
+components from the FP32/FP16 run at the end (FP16 Cb quantisation
+included).  ``--no-strip`` / ``--no-waveguide`` / ``--no-sweep`` /
+``--no-fp64`` skip those parts.
 
 ``--surrogate`` runs only the surrogate-screened coupler search (4096
 FDTD training spectra, a 4-member BF16 MLP ensemble, 65536 screened
@@ -72,6 +77,57 @@ def algorithmic_bytes(plan) -> float:
     for a in range(3):
         slab += (plan.pml[2 * a] + plan.pml[2 * a + 1]) * cells // n[a]
     return ALG_BYTES_PER_CELL * cells + ALG_BYTES_PER_PML_CELL_AXIS * slab
+
+
+def voxel_fp64(n, steps, warmup, ref):
+    """The same voxel workload with FP64 fields and FP64 coefficients (the
+    plain method), the same number of steps: throughput, the fused kernel's
+    roofline (120 B per cell + 64 B per CPML cell-axis), and the relative
+    L2 distance over all six components between the FP32/FP16 fields
+    ``ref`` and these, at the end of the run (it includes the FP16
+    quantisation of Cb, not only FP32 arithmetic)."""
+    import numpy as np
+    import torch
+
+    from prismo_amd.workloads import voxel_3d
+
+    sim = voxel_3d(dtype=np.float64, coefficient_storage="auto", n=n)
+    eng = sim.solver.updater.engine_for(sim.fields)
+    tuned = sim.autotune()
+    sim.run_steps(warmup)
+    torch.cuda.synchronize()
+    eng.ctx.set_timing(True, stride=max(1, min(10, steps // 20)))
+    t0 = time.perf_counter()
+    sim.run_steps(steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    f_ms, nf = eng.ctx.timing(3)
+    eng.ctx.set_timing(False)
+    p = eng.plan
+    cells = p.nx * p.ny * p.nz
+    slab = sum((p.pml[2 * a] + p.pml[2 * a + 1]) * cells // (p.nx, p.ny, p.nz)[a]
+               for a in range(3))
+    alg = 120 * cells + 64 * slab
+    num = den = 0.0
+    for c in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
+        f = sim.fields[c].double()
+        num += float(((f - ref[c].double()) ** 2).sum())
+        den += float((f ** 2).sum())
+    out = {"gcells_per_s": round(cells * steps / el / 1e9, 3),
+           "ms_per_step": round(el / steps * 1e3, 5),
+           "fused": bool(nf),
+           "launch": {k: tuned.get(k) for k in ("vec", "ty", "kchunk", "order")},
+           "rel_l2_fp32_fp16_vs_fp64_6comp": (num / den) ** 0.5 if den > 0 else None}
+    if nf:
+        out["roofline"] = {"bound": "hbm", "kernel": "k_yee_fused<double>",
+                           "achieved": round(alg / (f_ms * 1e-3) / 1e9, 1),
+                           "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(alg / (f_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                           "alg_bytes_per_launch": alg, "kernel_ms": round(f_ms, 5)}
+    del sim, eng
+    torch.cuda.empty_cache()
+    print(f"[bench] FP64 voxel: {out}", file=sys.stderr, flush=True)
+    return out
 
 
 def measured_traffic(workload_key: str, launch: dict):
@@ -372,6 +428,7 @@ def main():
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--batch-probe", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--surrogate", action="store_true")
     ap.add_argument("--sweep-only", action="store_true")
     ap.add_argument("--no-strip", action="store_true")
@@ -449,10 +506,14 @@ def main():
     wkey = f"voxel_3d_{args.n}_fp32_fp16"
     traffic = measured_traffic(wkey, launch) if nf >= nh else None
     finite = all(bool(torch.isfinite(sim.fields[c]).all()) for c in ("Ex", "Ey", "Ez"))
+    ref = {c: sim.fields[c].clone() for c in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")}
     print(f"[bench] {args.steps} steps in {el:.3f} s, fused {f_ms:.4f} ms x{nf}, "
           f"H {h_ms:.4f} ms x{nh}, E {e_ms:.4f} ms x{ne}, "
           f"finite={finite}", file=sys.stderr, flush=True)
     del sim, eng
+    torch.cuda.empty_cache()
+    fp64 = None if args.no_fp64 else voxel_fp64(args.n, args.steps, args.warmup, ref)
+    del ref
     torch.cuda.empty_cache()
     sweep = None if args.no_sweep else sweep_bench()
     strip = None if args.no_strip else strip_bench()
@@ -494,6 +555,7 @@ def main():
             "kernel_ms": round(kern_ms, 5),
         },
         "cpu_baseline": cpu,
+        "voxel_fp64": fp64,
         "sweep": sweep,
         "strip_2d": strip,
         "waveguide_3d": waveguide,
