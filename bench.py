@@ -321,6 +321,62 @@ def sweep_bench(n_designs=256, steps=4000, seq_sample=12, group=64, first=16, ro
     }
 
 
+def _cpu_coupler(args):
+    """One coupler design through the O0 NumPy restatement of the
+    reference's step (its arithmetic, one core) with the design's plane-wave
+    source and its three 64-frequency port monitors updated every step, as
+    the reference's Simulation.run does; returns the seconds ``steps`` steps
+    took."""
+    k, steps = args
+    import numpy as np
+
+    from oracle import compat as O0
+    from prismo_amd import workloads as W
+
+    p = W.coupler_designs(256)[k]
+    sim = W.coupler_sim(p, dtype=np.float64, backend=W.host_backend())
+    nx, ny, nz = sim.grid.dimensions
+    eps = np.asarray(sim.solver.updater.eps_rel).reshape(nx, ny, nz)
+    one, zero = np.ones_like(eps), np.zeros_like(eps)
+    C = dict(zip(("Ca", "Cb", "Da", "Db"), O0.coefficients(eps, one, zero, zero, sim.dt)))
+    F = O0.zeros(nx, ny, nz, True)
+    src = sim.sources[0]
+    dt = sim.dt
+    mons = [O0.RegionDFT({c: m._monitor_region[c] for c in m.components}, m.frequencies)
+            for m in sim.monitors]
+    t0 = time.perf_counter()
+    O0.run(F, C, sim.grid.spacing, True, dt, steps,
+           sources=[lambda F_, t: src.update_fields(F_, t, dt)], monitors=mons)
+    return time.perf_counter() - t0
+
+
+def cpu_sweep_baseline(steps=400, full_steps=4000, max_workers=16):
+    """The sweep the reference runs on the host (ParameterSweep with a process
+    pool, src/prismo/optimization/sweep.py:144-167): one process per core
+    (at most 16), each running one of the 256 coupler designs (evenly
+    spaced) for ``steps`` steps of the O0 reference-arithmetic step; the
+    throughput is scaled to ``full_steps``-step designs.  Run before the GPU
+    is touched (the workers are forked)."""
+    from concurrent.futures import ProcessPoolExecutor
+
+    import numpy as np
+
+    workers = max(1, min(max_workers, os.cpu_count() or 1))
+    ks = [int(k) for k in np.linspace(0, 255, workers).round()]
+    t0 = time.perf_counter()
+    with ProcessPoolExecutor(max_workers=workers) as ex:
+        secs = list(ex.map(_cpu_coupler, [(k, steps) for k in ks]))
+    wall = time.perf_counter() - t0
+    per_design = float(np.mean(secs)) * full_steps / steps
+    out = {"designs_per_s": round(workers / per_design, 4), "unit": "designs/s",
+           "cores": workers, "kind": "port",
+           "sample": f"{workers} designs x {steps} steps, one per process (O0 NumPy "
+                     f"reference-arithmetic step, plane-wave source, 3 x 64-frequency port "
+                     f"DFTs per step), scaled to {full_steps} steps; {wall:.1f} s wall"}
+    print(f"[bench] CPU sweep sample: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def strip_bench(cpu_steps=60):
     """2-D TM strip (540 x 290 total cells, FP64, 3000 steps with an
     11-frequency line monitor): the reference-arithmetic ('compat') and the
@@ -446,6 +502,12 @@ def main():
         batch_probe()
         return
 
+    # the host-process sweep baseline first, while no process has touched
+    # the GPU (its workers are forked from this one)
+    runs_sweep = not (args.no_sweep or args.waveguide_only or args.surrogate)
+    cpu_sweep = (cpu_sweep_baseline() if runs_sweep and not args.no_cpu_baseline
+                 else None)
+
     import numpy as np
     import torch
 
@@ -454,7 +516,12 @@ def main():
 
     pa.set_backend("mi355x", 0)
     if args.sweep_only:
-        print(json.dumps({"sweep": sweep_bench()}), flush=True)
+        sweep = sweep_bench()
+        if cpu_sweep is not None:
+            sweep["cpu_baseline"] = cpu_sweep
+            sweep["speedup_vs_cpu_pool"] = round(sweep["designs_per_s"]
+                                                 / cpu_sweep["designs_per_s"], 1)
+        print(json.dumps({"sweep": sweep}), flush=True)
         return
     if args.waveguide_only:
         print(json.dumps({"waveguide_3d": waveguide_bench()}), flush=True)
@@ -516,6 +583,9 @@ def main():
     del ref
     torch.cuda.empty_cache()
     sweep = None if args.no_sweep else sweep_bench()
+    if sweep is not None and cpu_sweep is not None:
+        sweep["cpu_baseline"] = cpu_sweep
+        sweep["speedup_vs_cpu_pool"] = round(sweep["designs_per_s"] / cpu_sweep["designs_per_s"], 1)
     strip = None if args.no_strip else strip_bench()
     waveguide = None if args.no_waveguide else waveguide_bench()
     cpu = None
