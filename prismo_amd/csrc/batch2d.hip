@@ -72,9 +72,11 @@ __device__ __forceinline__ T ldg(const void* base, uint32_t e) {
   return *(gptr<const T>)((gptr<const char>)base + e * (uint32_t)sizeof(T));
 }
 
+// Stores are non-temporal: the other field set is read one step later,
+// after every design's step, so caching it only displaces reused lines.
 template <typename T>
 __device__ __forceinline__ void stg(void* base, uint32_t e, T v) {
-  *(gptr<T>)((gptr<char>)base + e * (uint32_t)sizeof(T)) = v;
+  __builtin_nontemporal_store(v, (gptr<T>)((gptr<char>)base + e * (uint32_t)sizeof(T)));
 }
 
 // Value of v in lane l (l uniform), for the whole wave.

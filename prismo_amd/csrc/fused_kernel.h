@@ -144,16 +144,19 @@ __device__ __forceinline__ void fld(T (&d)[VEC], const T* p) {
   }
 }
 
+// Stores of the output field set are non-temporal: nothing reads them before
+// the next step, a whole field set of traffic later, so they should not
+// displace the read-only coefficients from the caches.
 template <typename T, int VEC>
 __device__ __forceinline__ void fst(T* p, const T (&s)[VEC]) {
   if constexpr (VEC == 1) {
-    *p = s[0];
+    __builtin_nontemporal_store(s[0], p);
   } else {
     using VT = typename FVec<T, VEC>::type;
     VT v;
 #pragma unroll
     for (int e = 0; e < VEC; ++e) v[e] = s[e];
-    *(VT*)p = v;
+    __builtin_nontemporal_store(v, (VT*)p);
   }
 }
 
