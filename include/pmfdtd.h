@@ -225,10 +225,24 @@ int pm_update_e(pm_ctx* ctx);
 #define PM_RUN_SAMPLE  2u
 #define PM_RUN_DFT     4u
 #define PM_RUN_GRAPH   8u   /* replay the step loop as captured HIP graphs */
+/* Take no step: sample step step0 (nsteps must be 1) from the fields as
+ * they are, with PM_RUN_SAMPLE / PM_RUN_DFT as above.  For callers that add
+ * terms of their own after pm_run's step and sources (src/prismo/core/
+ * simulation.py:158-164: every source before any monitor). */
+#define PM_RUN_IO_ONLY 16u
 int pm_run(pm_ctx* ctx, int64_t step0, int32_t nsteps, uint32_t flags);
 
-/* Accumulate the first `rows` rows of X, which hold steps
- * [step_first, step_first + rows), into every DFT group (partial block). */
+/* Spectral flush outside pm_run.
+ *  rows == 0: sum the samples pm_run has taken but not yet summed (the
+ *    partial block a run leaves when it ends mid-block); step_first is not
+ *    used.
+ *  rows > 0: sum rows (step_first mod sample_T) .. + rows - 1 of X as the
+ *    samples of steps [step_first, step_first + rows) (phasor columns
+ *    step_first - table_step0 ..), e.g. samples the caller wrote into X
+ *    itself.  The rows must not run past the end of X (PM_EINVAL) and the
+ *    phasor tables must cover the steps (PM_ERANGE); a range equal to pm_run's
+ *    pending samples sums and clears them, one that only overlaps them is
+ *    refused (PM_EINVAL). */
 int pm_dft_flush(pm_ctx* ctx, int64_t step_first, int32_t rows);
 
 /* Launch shape of the fused step: rows per workgroup (ty in {2, 3, 4}), z planes

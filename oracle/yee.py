@@ -117,20 +117,39 @@ class YeeOracle:
 
     # ---------------------------------------------------------------- helpers
     def _term(self, key, d_arr, axis, side):
-        """CPML-stretched derivative term (side 'e' or 'h')."""
+        """CPML-stretched derivative term (side 'e' or 'h'):
+        psi <- b psi + c d, term = d / kappa + psi.  Outside the layers
+        b = 1, c = 0, 1/kappa = 1 and psi stays 0, so the term is d itself;
+        only the layer slabs along ``axis`` are computed (same values as the
+        full-array form)."""
         if self.prof is None or axis >= len(self.prof) or self.prof[axis] is None:
             return d_arr
         pr = self.prof[axis]
         nd = d_arr.ndim
-        b = _bc(pr["b" + side][: d_arr.shape[axis]], axis, nd)
-        c = _bc(pr["c" + side][: d_arr.shape[axis]], axis, nd)
-        ik = _bc(pr["ik" + side][: d_arr.shape[axis]], axis, nd)
-        psi = self.psi.get(key)
-        if psi is None:
-            psi = np.zeros_like(d_arr)
-        psi = b * psi + c * d_arr
-        self.psi[key] = psi
-        return d_arr * ik + psi
+        L = d_arr.shape[axis]
+        tl, th = self.pml[2 * axis], self.pml[2 * axis + 1]
+        n = self.n[axis]
+        slabs = [s for s in ((0, min(tl, L)), (max(n - th, 0), L)) if s[1] > s[0]]
+        psis = self.psi.get(key)
+        if psis is None:
+            psis = []
+            for a0, a1 in slabs:
+                shp = list(d_arr.shape)
+                shp[axis] = a1 - a0
+                psis.append(np.zeros(shp, dtype=d_arr.dtype))
+            self.psi[key] = psis
+        out = d_arr.copy()
+        for (a0, a1), psi in zip(slabs, psis):
+            sl = [slice(None)] * nd
+            sl[axis] = slice(a0, a1)
+            sl = tuple(sl)
+            b = _bc(pr["b" + side][a0:a1], axis, nd)
+            c = _bc(pr["c" + side][a0:a1], axis, nd)
+            ik = _bc(pr["ik" + side][a0:a1], axis, nd)
+            d = d_arr[sl]
+            psi[...] = b * psi + c * d
+            out[sl] = d * ik + psi
+        return out
 
     def add_ade(self, comp, box, kind, eps_inf, c):
         (i0, i1), (j0, j1), (k0, k1) = box
@@ -220,6 +239,22 @@ class YeeOracle:
     def step(self):
         self.update_h()
         self.update_e()
+
+    def run(self, nsteps, dt=None, t0=0.0, sources=(), monitors=()):
+        """Simulation.step loop (src/prismo/core/simulation.py:147-164): H,
+        E, ``t += dt`` accumulated in FP64, every source at the new time,
+        then every monitor.  sources: callables (F, t) -> None; monitors:
+        objects with .update(F, t, dt).  Returns the final time."""
+        dt = self.dt if dt is None else dt
+        t = t0
+        for _ in range(nsteps):
+            self.step()
+            t += dt
+            for s in sources:
+                s(self.F, t)
+            for m in monitors:
+                m.update(self.F, t, dt)
+        return t
 
 
 def add_box(F, comp, box, value, is_2d):

@@ -21,7 +21,7 @@ COMP_ID = {"Ex": 0, "Ey": 1, "Ez": 2, "Hx": 3, "Hy": 4, "Hz": 5}
 SCHEME_COMPAT, SCHEME_YEE = 0, 1
 FP64, FP32 = 0, 1
 COEFF_FULL, COEFF_CB, COEFF_CB_F16 = 0, 1, 2
-RUN_SOURCES, RUN_SAMPLE, RUN_DFT, RUN_GRAPH = 1, 2, 4, 8
+RUN_SOURCES, RUN_SAMPLE, RUN_DFT, RUN_GRAPH, RUN_IO_ONLY = 1, 2, 4, 8, 16
 
 ERRORS = {
     -1: "invalid argument",
@@ -262,8 +262,10 @@ class Context:
     def run(self, step0: int, nsteps: int, flags: int):
         self._check(self.lib.pm_run(self.h, step0, nsteps, flags), "pm_run")
 
-    def dft_flush(self):
-        self._check(self.lib.pm_dft_flush(self.h, 0, 0), "pm_dft_flush")
+    def dft_flush(self, step_first: int = 0, rows: int = 0):
+        """rows == 0: sum the samples pm_run has not summed yet; rows > 0:
+        sum rows of X as the samples of steps [step_first, +rows)."""
+        self._check(self.lib.pm_dft_flush(self.h, step_first, rows), "pm_dft_flush")
 
     def set_tuning(self, fuse: bool = True, ty: int = 4, kchunk: int = 16, vec: int = 0,
                    order: int = 0):

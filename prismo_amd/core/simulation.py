@@ -187,8 +187,18 @@ class Simulation:
                 i = n - step0
                 nxt = i if i % progress_interval == 0 else i + (progress_interval - i % progress_interval)
                 seg = min(seg, nxt - i + 1)
-            eng.run(n, seg, flags)
             seg_times = times_arr[n - step0:n - step0 + seg]
+            if py_sources:
+                # every source before any monitor (simulation.py:158-164):
+                # step with the device sources, add the host ones, then
+                # sample the device monitors from those fields
+                eng.run(n, seg, N.RUN_SOURCES)
+                for s in py_sources:
+                    s.update_fields(self.fields, float(seg_times[-1]), dt)
+                if dev_mons:
+                    eng.run(n, 1, N.RUN_SAMPLE | N.RUN_DFT | N.RUN_IO_ONLY)
+            else:
+                eng.run(n, seg, flags)
             if any_td:
                 eng.sync()
                 rows = eng.sample_rows(n % T, seg)
@@ -202,8 +212,6 @@ class Simulation:
             self.solver.time = self.current_time
             self.solver.step_count = self.step_count
             if per_step:
-                for s in py_sources:
-                    s.update_fields(self.fields, self.current_time, dt)
                 for m in py_mons:
                     m.update(self.fields, self.current_time, dt)
             if progress_callback is not None:

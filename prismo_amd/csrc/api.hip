@@ -373,21 +373,29 @@ bool fused_eligible(const pm_plan& p) {
   return true;
 }
 
-int flush_pending(pm_ctx* c) {
+// Sum rows (first mod T) .. (first mod T) + rows - 1 of X, the samples of
+// steps [first, first + rows), into every DFT group.
+int flush_rows(pm_ctx* c, int64_t first, int32_t rows) {
   const pm_plan& p = c->plan;
-  if (c->pend_n == 0) return 0;
-  const int64_t wcol0 = c->pend0 - p.table_step0;
-  if (wcol0 < 0 || wcol0 + c->pend_n > p.table_len)
+  if (rows <= 0) return 0;
+  const int64_t wcol0 = first - p.table_step0;
+  if (wcol0 < 0 || wcol0 + rows > p.table_len)
     return fail(c, PM_ERANGE, "phasor table does not cover steps %lld..%lld",
-                (long long)c->pend0, (long long)(c->pend0 + c->pend_n - 1));
+                (long long)first, (long long)(first + rows - 1));
   const int esz = p.precision == PM_FP64 ? 8 : 4;
-  const char* X0 = (const char*)p.X + (c->pend0 % p.sample_T) * (int64_t)p.ncols * esz;
+  const char* X0 = (const char*)p.X + (first % p.sample_T) * (int64_t)p.ncols * esz;
   Span sp(c, 4);
   for (const KDft& d : c->dft)
-    launch_dft(d, X0, p.ncols, p.W_re, p.W_im, p.table_len, wcol0, c->pend_n, p.precision,
+    launch_dft(d, X0, p.ncols, p.W_re, p.W_im, p.table_len, wcol0, rows, p.precision,
                c->stream);
-  c->pend_n = 0;
   return launch_status(c, "DFT launch");
+}
+
+int flush_pending(pm_ctx* c) {
+  if (c->pend_n == 0) return 0;
+  const int r = flush_rows(c, c->pend0, c->pend_n);
+  if (r == 0) c->pend_n = 0;
+  return r;
 }
 
 }  // namespace
@@ -601,21 +609,59 @@ int pm_update_e(pm_ctx* c) {
   return do_e(c);
 }
 
+namespace {
+
+// Sample step n from the current state (after its sources) and sum a
+// completed block; a gap in the sampled steps sums (or drops) the block.
+int sample_step(pm_ctx* c, int64_t n, bool dft) {
+  const pm_plan& p = c->plan;
+  int r;
+  if (c->pend_n && n != c->pend0 + c->pend_n) {
+    if (dft) {
+      if ((r = flush_pending(c))) return r;
+    } else {
+      c->pend_n = 0;
+    }
+  }
+  const KAux& ax = c->parity ? c->aux2 : c->aux;
+  launch_sample(ax, n, c->stream);
+  if ((r = launch_status(c, "sample launch"))) return r;
+  if (c->pend_n == 0) c->pend0 = n;
+  ++c->pend_n;
+  if ((n + 1) % p.sample_T == 0) {
+    if (dft) {
+      if ((r = flush_pending(c))) return r;
+    } else {
+      c->pend_n = 0;
+    }
+  }
+  return 0;
+}
+
+}  // namespace
+
 int pm_run(pm_ctx* c, int64_t step0, int32_t nsteps, uint32_t flags) {
   if (!c) return PM_EINVAL;
   if (!c->configured) return fail(c, PM_ENOPLAN, "pm_run before pm_configure");
   if (nsteps < 0 || step0 < 0) return fail(c, PM_EINVAL, "negative step range");
   const pm_plan& p = c->plan;
-  const bool src = (flags & PM_RUN_SOURCES) && p.n_src > 0;
+  const bool io_only = (flags & PM_RUN_IO_ONLY) != 0;
+  const bool src = (flags & PM_RUN_SOURCES) && p.n_src > 0 && !io_only;
   const bool smp = (flags & PM_RUN_SAMPLE) && p.n_sbox > 0;
   const bool dft = (flags & PM_RUN_DFT) && smp && !c->dft.empty();
   if (flags & PM_RUN_GRAPH) return fail(c, PM_ENOTSUP, "graph replay is not built");
+  if (io_only && nsteps > 1)
+    return fail(c, PM_EINVAL, "PM_RUN_IO_ONLY samples one step (nsteps %d)", nsteps);
   if (src || dft) {
     const int64_t r0 = step0 - p.table_step0;
     if (r0 < 0 || r0 + nsteps > p.table_len)
       return fail(c, PM_ERANGE, "tables cover steps [%lld, %lld), run asks [%lld, %lld)",
                   (long long)p.table_step0, (long long)(p.table_step0 + p.table_len),
                   (long long)step0, (long long)(step0 + nsteps));
+  }
+  if (io_only) {
+    if (nsteps == 1 && smp) return sample_step(c, step0, dft);
+    return 0;
   }
   int fvec = 0, fw = 0, fty = 0;
   const bool fused = c->fused_ok && c->fuse && fused_shape(c, &fvec, &fw, &fty);
@@ -638,39 +684,39 @@ int pm_run(pm_ctx* c, int64_t step0, int32_t nsteps, uint32_t flags) {
       launch_sources(ax, n, c->stream);
       if ((r = launch_status(c, "source launch"))) return r;
     }
-    if (smp) {
-      if (c->pend_n && n != c->pend0 + c->pend_n) {
-        if (dft) {
-          if ((r = flush_pending(c))) return r;
-        } else {
-          c->pend_n = 0;
-        }
-      }
-      launch_sample(ax, n, c->stream);
-      if ((r = launch_status(c, "sample launch"))) return r;
-      if (c->pend_n == 0) c->pend0 = n;
-      ++c->pend_n;
-      if ((n + 1) % p.sample_T == 0) {
-        if (dft) {
-          if ((r = flush_pending(c))) return r;
-        } else {
-          c->pend_n = 0;
-        }
-      }
-    }
+    if (smp && (r = sample_step(c, n, dft))) return r;
   }
   c->span_on = c->timing;
   return 0;
 }
 
-int pm_dft_flush(pm_ctx* c, int64_t /*step_first*/, int32_t /*rows*/) {
+int pm_dft_flush(pm_ctx* c, int64_t step_first, int32_t rows) {
   if (!c) return PM_EINVAL;
   if (!c->configured) return fail(c, PM_ENOPLAN, "pm_dft_flush before pm_configure");
-  if (c->dft.empty()) {
-    c->pend_n = 0;
-    return 0;
+  if (rows < 0 || step_first < 0) return fail(c, PM_EINVAL, "negative step or row count");
+  if (rows == 0) {
+    if (c->dft.empty()) {
+      c->pend_n = 0;
+      return 0;
+    }
+    return flush_pending(c);
   }
-  return flush_pending(c);
+  const pm_plan& p = c->plan;
+  if (c->dft.empty()) return fail(c, PM_EINVAL, "pm_dft_flush: the plan has no DFT groups");
+  if (step_first % p.sample_T + rows > p.sample_T)
+    return fail(c, PM_EINVAL, "rows %lld..%lld run past the %d rows of X",
+                (long long)(step_first % p.sample_T),
+                (long long)(step_first % p.sample_T + rows - 1), p.sample_T);
+  if (c->pend_n) {
+    const int64_t a0 = c->pend0, a1 = c->pend0 + c->pend_n;
+    if (step_first == a0 && rows == c->pend_n) return flush_pending(c);
+    if (step_first < a1 && a0 < step_first + rows)
+      return fail(c, PM_EINVAL,
+                  "steps %lld..%lld overlap samples pm_run has not summed (%lld..%lld)",
+                  (long long)step_first, (long long)(step_first + rows - 1), (long long)a0,
+                  (long long)(a1 - 1));
+  }
+  return flush_rows(c, step_first, rows);
 }
 
 int pm_set_tuning(pm_ctx* c, int fuse, int ty, int kchunk, int vec, int order) {

@@ -192,7 +192,8 @@ WG_DIMS = (480, 240, 160)
 WG_PML = 12
 
 
-def waveguide_3d(dtype=np.float32, coefficient_storage="auto", steps=WG_STEPS, backend=None):
+def waveguide_3d(dtype=np.float32, coefficient_storage="auto", steps=WG_STEPS, backend=None,
+                 dims=WG_DIMS, pulse_width=10e-15):
     """Dispersive silicon waveguide, 480 x 240 x 160 total cells at 20 nm
     (12-cell CPML inside): a 450 x 220 nm Lorentz-silicon core along x
     (eps_inf 7.98, delta-eps 3.69, w0 1.1e16, gamma 1e13) on an SiO2
@@ -202,19 +203,36 @@ def waveguide_3d(dtype=np.float32, coefficient_storage="auto", steps=WG_STEPS, b
     mode (solved at 193 THz with the core's Lorentz eps) is launched along +x
     from x = 1 um with a 193 THz Gaussian pulse (10 fs); the y-z plane at
     x = 8.6 um records Ey, Ez, Hy, Hz at 64 frequencies (180-210 THz).
-    Returns (sim, monitor, mode, steps)."""
+
+    ``dims`` other than 480 x 240 x 160 give the same structure on a smaller
+    grid (same CPML, core cross-section and materials): the substrate top,
+    the pad box and the source and monitor planes keep their fractions of
+    the physical lengths.  Returns (sim, monitor, mode, steps)."""
     import prismo_amd as pa
 
-    nx, ny, nz = WG_DIMS
     p = WG_PML
-    size = tuple((n - 2 * p) * D * SHRINK for n in WG_DIMS)
+    size = tuple((n - 2 * p) * D * SHRINK for n in dims)
+    full = tuple(dims) == WG_DIMS
+    ref = tuple((n - 2 * p) * D for n in WG_DIMS)
+
+    def at(v, ax):
+        """A full-size position (from the lower physical edge) on this grid."""
+        return v if full else v * (size[ax] / ref[ax])
+
     sim = pa.Simulation(size=size, resolution=1 / D, pml_layers=p, courant_factor=0.99,
                         scheme="yee", dtype=dtype, backend=backend,
                         pml=pa.PMLParams(thickness=p), coefficient_storage=coefficient_storage)
     yc = size[1] / 2
-    z_sub = 1.2e-6                     # substrate top
+    z_sub = at(1.2e-6, 2)              # substrate top
     core = ((0.0, yc - 225e-9, z_sub), (size[0], yc + 225e-9, z_sub + 220e-9))
-    pad = ((4.0e-6, yc - 2.0e-6, z_sub + 520e-9), (8.0e-6, yc + 2.0e-6, z_sub + 1.52e-6))
+    if full:
+        pz = (z_sub + 520e-9, z_sub + 1.52e-6)
+    else:
+        # the core keeps its 220 nm: the gap above it scales, the pad ends
+        # at the top of the physical domain as at full size
+        pz = (z_sub + 220e-9 + at(300e-9, 2), at(2.72e-6, 2))
+    pad = ((at(4.0e-6, 0), yc - at(2.0e-6, 1), pz[0]),
+           (at(8.0e-6, 0), yc + at(2.0e-6, 1), pz[1]))
     si = pa.LorentzMaterial(7.98, [pa.LorentzPole(omega_0=1.1e16, delta_epsilon=3.69,
                                                   gamma=1e13)], name="Si")
     au = pa.DrudeMaterial(1.0, omega_p=1.37e16, gamma=1.07e14, name="pad")
@@ -228,21 +246,24 @@ def waveguide_3d(dtype=np.float32, coefficient_storage="auto", steps=WG_STEPS, b
         sim.add_structure(s)
     sim.add_dispersive_box(si, core[0], core[1])
     sim.add_dispersive_box(au, pad[0], pad[1])
+    sim.wg_shapes = shapes
+    sim.wg_materials = (si, au)
 
     f0 = 193e12
     g = sim.grid
     y = g.origin[1] + np.arange(g.dimensions[1]) * g.dy
     z = g.origin[2] + np.arange(g.dimensions[2]) * g.dz
     Y, Z = np.meshgrid(y, z, indexing="ij")
-    pts = np.column_stack([np.full(Y.size, 1e-6), Y.ravel(), Z.ravel()])
+    pts = np.column_stack([np.full(Y.size, at(1e-6, 0)), Y.ravel(), Z.ravel()])
     eps = np.ones(Y.size)
     eps[shapes[0].contains(pts)] = 2.085
     eps[shapes[1].contains(pts)] = float(np.real(si.permittivity(2 * np.pi * f0)))
     mode = pa.ModeSolver(299792458.0 / f0, y, z, eps.reshape(Y.shape)).solve(1, "TE")[0]
-    wf = pa.sources.waveform.GaussianPulse(f0, 10e-15)
-    sim.add_source(pa.ModeSource(center=(1e-6, yc, size[2] / 2), size=(0.0, size[1], size[2]),
-                                 mode=mode, direction="+x", waveform=wf))
-    mon = pa.FieldMonitor(center=(8.6e-6, yc, size[2] / 2), size=(0.0, size[1] + 2 * p * D,
+    wf = pa.sources.waveform.GaussianPulse(f0, pulse_width)
+    sim.add_source(pa.ModeSource(center=(at(1e-6, 0), yc, size[2] / 2),
+                                 size=(0.0, size[1], size[2]), mode=mode, direction="+x",
+                                 waveform=wf))
+    mon = pa.FieldMonitor(center=(at(8.6e-6, 0), yc, size[2] / 2), size=(0.0, size[1] + 2 * p * D,
                                                                     size[2] + 2 * p * D),
                           components=["Ey", "Ez", "Hy", "Hz"], time_domain=False,
                           frequencies=list(WG_FREQS))

@@ -259,9 +259,20 @@ def test_yee_3d_cpml(mi355x, dtype):
     sim.run(steps * sim.dt * 0.9999)
     o1 = _yee_oracle(sim, eps, t)
     ref = _run_o1(o1, sim, steps, mon)
-    tol = 1e-10 if dtype == "float64" else 2e-5
-    for c in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
-        assert rel_l2(sim.fields.to_numpy(c), o1.F[c]) <= tol, c
-    for c in mon.components:
-        for f in mon.frequencies:
-            assert rel_l2(mon.get_frequency_data(c, f), ref.data[c][f]) <= (1e-10 if dtype == "float64" else 1e-4)
+    # FP32 fields against the FP64 oracle fed the same FP64 coefficients:
+    # the north star's 1e-5 bar on every component and every spectrum
+    tol = 1e-10 if dtype == "float64" else 1e-5
+    comps = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
+    errs = {c: rel_l2(sim.fields.to_numpy(c), o1.F[c]) for c in comps}
+    whole = rel_l2(np.concatenate([sim.fields.to_numpy(c).ravel() for c in comps]),
+                   np.concatenate([o1.F[c].ravel() for c in comps]))
+    spec = {(c, f): rel_l2(mon.get_frequency_data(c, f), ref.data[c][f])
+            for c in mon.components for f in mon.frequencies}
+    print(f"yee 3-D {dtype}: six-component {whole:.3e}; per component "
+          + " ".join(f"{c} {e:.2e}" for c, e in errs.items())
+          + "; spectra max " + f"{max(spec.values()):.2e}")
+    assert whole <= tol
+    for c, e in errs.items():
+        assert e <= tol, (c, e)
+    for k, e in spec.items():
+        assert e <= tol, (k, e)
