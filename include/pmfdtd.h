@@ -210,6 +210,18 @@ int pm_abi_version(void);
 /* Validate and adopt a plan (deep-copies the host arrays it points to). */
 int pm_configure(pm_ctx* ctx, const pm_plan* plan);
 
+/* pm_configure for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  plan->scratch must be NULL: the library
+ * uses its own PM_SCRATCH_BYTES (4 KiB) on the context's device instead (at
+ * most 8 dispersive boxes, 24 monitor boxes and 16 sources, as for
+ * pm_configure; more is refused with PM_EINVAL).  Allocates no device memory.
+ * That scratch is one array per device, shared by every context configured
+ * this way on it and keeping the last configure's tables: such contexts on
+ * one device must not be configured or run concurrently (they race), and a
+ * context must be configured again after another one was. */
+int pm_configure_static(pm_ctx* ctx, const pm_plan* plan);
+
 /* One H half-step (MaxwellUpdater.update_magnetic_fields,
  * solver.py:135-149) / one E half-step (update_electric_fields, 151-165).
  * No sources or monitors. */

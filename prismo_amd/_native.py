@@ -182,6 +182,7 @@ SIGNATURES = {
     "pm_destroy": ([vp], C.c_int),
     "pm_last_error": ([vp], C.c_char_p),
     "pm_configure": ([vp, C.POINTER(pm_plan)], C.c_int),
+    "pm_configure_static": ([vp, C.POINTER(pm_plan)], C.c_int),
     "pm_update_h": ([vp], C.c_int),
     "pm_update_e": ([vp], C.c_int),
     "pm_run": ([vp, i64, i32, C.c_uint32], C.c_int),
@@ -252,6 +253,12 @@ class Context:
     def configure(self, plan: pm_plan, keep=()):
         self._keep = list(keep)  # host arrays referenced by the plan
         self._check(self.lib.pm_configure(self.h, C.byref(plan)), "pm_configure")
+
+    def configure_static(self, plan: pm_plan, keep=()):
+        """pm_configure_static: the library's own per-device scratch instead
+        of plan.scratch (which must be NULL); allocates no device memory."""
+        self._keep = list(keep)
+        self._check(self.lib.pm_configure_static(self.h, C.byref(plan)), "pm_configure_static")
 
     def update_h(self):
         self._check(self.lib.pm_update_h(self.h), "pm_update_h")

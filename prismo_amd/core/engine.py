@@ -88,7 +88,9 @@ class StepEngine:
         self._fill_cpml(cpml)
         self._fill_ade(list(ade))
         self._fill_fused(fused)
-        self.plan.scratch = self._alloc((N.SCRATCH_BYTES // 4,), torch.int32).data_ptr()
+        self._scratch_ptr = self._alloc((N.SCRATCH_BYTES // 4,), torch.int32).data_ptr()
+        self.plan.scratch = self._scratch_ptr
+        self.static_scratch = False
         stream = torch.cuda.current_stream(self.device)
         self.ctx = N.Context(self.device.index or 0, stream.cuda_stream)
         self._io_keep: list = []
@@ -516,7 +518,20 @@ class StepEngine:
             p.W_re, p.W_im = Wre.data_ptr(), Wim.data_ptr()
         else:
             p.W_re = p.W_im = None
-        self.ctx.configure(self.plan)
+        if self.static_scratch:
+            p.scratch = None
+            self.ctx.configure_static(self.plan)
+        else:
+            p.scratch = self._scratch_ptr
+            self.ctx.configure(self.plan)
+
+    def use_static_scratch(self, on: bool = True):
+        """Configure through pm_configure_static (the library's per-device
+        scratch; no device allocation in the configure call) from now on.
+        Every engine doing so on one device shares that scratch: configure
+        and run such engines one after another, never interleaved."""
+        self.static_scratch = bool(on)
+        self.configure_io()
 
     # ---------------------------------------------------------------- stepping
     def update_h(self):

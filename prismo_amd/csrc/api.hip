@@ -3,6 +3,7 @@
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -38,6 +39,13 @@ struct pm_ctx {
   size_t pool_used = 0;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> spans[5];
 };
+
+// Scratch of pm_configure_static: one 4 KiB copy on every device the
+// library runs on (a __device__ array exists once per device), held until
+// the process ends.  Host code never passes the array's name to the device:
+// pm_create resolves each device's own address once, with that device
+// current, and every later use takes that address.
+static __device__ __align__(256) unsigned char scratch[PM_SCRATCH_BYTES];
 
 namespace {
 
@@ -88,6 +96,25 @@ struct Span {
     }
   }
 };
+
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};
+int g_scratch_err[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// Resolve device `dev`'s copy of `scratch` (dev must be current).
+int resolve_static_scratch(int dev) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] || g_scratch_err[dev]) return g_scratch_err[dev];
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    g_scratch_err[dev] = e != hipSuccess ? (int)e : (int)hipErrorInvalidSymbol;
+    return g_scratch_err[dev];
+  }
+  g_scratch_ptr[dev] = ptr;
+  return 0;
+}
 
 bool box_ok(int i0, int i1, int j0, int j1, int k0, int k1, const pm_plan& p) {
   return i0 >= 0 && j0 >= 0 && k0 >= 0 && i0 <= i1 && j0 <= j1 && k0 <= k1 &&
@@ -413,6 +440,9 @@ int pm_create(int device, void* stream, pm_ctx** out) {
   if (device < 0 || device >= ndev) return PM_EINVAL;
   e = hipSetDevice(device);
   if (e != hipSuccess) return (int)e;
+  // the device's copy of the static scratch (pm_configure_static); a
+  // failure here only disables that entry point
+  if (device < kMaxDevices) (void)resolve_static_scratch(device);
   pm_ctx* c = new pm_ctx();
   c->device = device;
   c->stream = (hipStream_t)stream;
@@ -595,6 +625,29 @@ int pm_configure(pm_ctx* c, const pm_plan* plan) {
   c->pend_n = 0;
   c->configured = true;
   return 0;
+}
+
+int pm_configure_static(pm_ctx* c, const pm_plan* plan) {
+  if (!c || !plan) return PM_EINVAL;
+  if (plan->scratch)
+    return fail(c, PM_EINVAL,
+                "pm_configure_static: plan.scratch must be NULL (the library's own "
+                "%d-byte scratch is used)", PM_SCRATCH_BYTES);
+  // what the scratch holds, against its fixed size (validate repeats these)
+  if (plan->n_ade > kMaxAde || plan->n_sbox > kMaxSbox || plan->n_src > kMaxSrc)
+    return fail(c, PM_EINVAL,
+                "pm_configure_static: %d dispersive boxes, %d monitor boxes, %d sources "
+                "exceed the %d-byte static scratch (at most %d, %d, %d)",
+                plan->n_ade, plan->n_sbox, plan->n_src, PM_SCRATCH_BYTES, kMaxAde, kMaxSbox,
+                kMaxSrc);
+  void* ptr = c->device < kMaxDevices ? g_scratch_ptr[c->device] : nullptr;
+  if (!ptr)
+    return fail(c, PM_ENOTSUP,
+                "pm_configure_static: no static scratch on device %d (symbol lookup status %d)",
+                c->device, c->device < kMaxDevices ? g_scratch_err[c->device] : -1);
+  pm_plan p = *plan;
+  p.scratch = ptr;
+  return pm_configure(c, &p);
 }
 
 int pm_update_h(pm_ctx* c) {
