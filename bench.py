@@ -95,6 +95,7 @@ def voxel_fp64(n, steps, warmup, ref):
     eng = sim.solver.updater.engine_for(sim.fields)
     tuned = sim.autotune()
     sim.run_steps(warmup)
+    sim.prepare(steps)
     torch.cuda.synchronize()
     eng.ctx.set_timing(True, stride=max(1, min(10, steps // 20)))
     t0 = time.perf_counter()
@@ -130,12 +131,58 @@ def voxel_fp64(n, steps, warmup, ref):
     return out
 
 
+ACC_STEPS = 1000
+
+
+def voxel_accuracy(n, tuned, steps=ACC_STEPS):
+    """The precision form against the plain method on the voxel workload
+    over a fixed 1000 steps, whatever the timed run's length: FP32 fields
+    with FP16 Cb (the timed launch shape) against FP64 fields fed the same
+    FP16-rounded Cb (arithmetic error only), and against FP64 with the exact
+    Cb (the quantisation included); relative L2 over all six components."""
+    import numpy as np
+    import torch
+
+    from prismo_amd.workloads import voxel_3d
+
+    comps = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
+    s32 = voxel_3d(dtype=np.float32, coefficient_storage="fp16", n=n)
+    e32 = s32.solver.updater.engine_for(s32.fields)
+    if tuned.get("fused"):
+        e32.ctx.set_tuning(tuned.get("steps", 1), tuned["ty"], tuned["kchunk"], tuned["vec"],
+                           tuned["order"])
+    s32.run_steps(steps)
+    f32 = {c: s32.fields[c].double() for c in comps}
+    cb_q = dict(e32.cb_host)     # the FP16-rounded Cb, in FP64
+    del s32, e32
+    torch.cuda.empty_cache()
+    out = {"steps": steps}
+    for tag, quant in (("vs_fp64_same_cb", True), ("vs_fp64_exact_cb", False)):
+        s64 = voxel_3d(dtype=np.float64, coefficient_storage="auto", n=n)
+        e64 = s64.solver.updater.engine_for(s64.fields)
+        if quant:
+            for c, v in cb_q.items():
+                e64._padded(v, c, torch.float64, buf=e64.cb_dev[c])
+        s64.run_steps(steps)
+        num = den = 0.0
+        for c in comps:
+            f = s64.fields[c]
+            num += float(((f32[c] - f) ** 2).sum())
+            den += float((f ** 2).sum())
+        out["rel_l2_" + tag] = (num / den) ** 0.5 if den > 0 else None
+        del s64, e64
+        torch.cuda.empty_cache()
+    print(f"[bench] voxel accuracy over {steps} steps: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def measured_traffic(workload_key: str, launch: dict):
     """HBM bytes per fused launch from the committed counter summary, when
     it covers this workload and launch shape (else None)."""
     if not os.path.exists(TRAFFIC_FILE):
         return None
     want = tuple(launch.get(k) for k in ("vec", "ty", "kchunk", "order"))
+    steps = launch.get("steps") or 1
     for line in open(TRAFFIC_FILE):
         if not line.startswith("traffic:"):
             continue
@@ -143,7 +190,8 @@ def measured_traffic(workload_key: str, launch: dict):
         try:
             if vals.get("workload") != workload_key:
                 continue
-            if tuple(int(x) for x in vals["launch"].split(",")) == want:
+            if (tuple(int(x) for x in vals["launch"].split(",")) == want
+                    and int(vals.get("steps", 1)) == steps):
                 return float(vals["bytes"])
         except (KeyError, ValueError, TypeError):
             continue
@@ -537,13 +585,19 @@ def main():
     eng = sim.solver.updater.engine_for(sim.fields)
     print(f"[bench] built {args.n}^3 FP32/FP16 voxel workload", file=sys.stderr, flush=True)
     if args.launch:
-        vec, ty, kc, order = (int(x) for x in args.launch.split(","))
-        eng.ctx.set_tuning(True, ty, kc, vec, order)
-        tuned = {"vec": vec, "ty": ty, "kchunk": kc, "order": order, "fused": True}
+        parts = [int(x) for x in args.launch.split(",")]
+        vec, ty, kc, order = parts[:4]
+        nstep = parts[4] if len(parts) > 4 else 1
+        eng.ctx.set_tuning(nstep, ty, kc, vec, order)
+        tuned = {"steps": nstep, "vec": vec, "ty": ty, "kchunk": kc, "order": order,
+                 "fused": True}
     else:
         tuned = sim.autotune()
     print(f"[bench] fused step launch: {tuned}", file=sys.stderr, flush=True)
     sim.run_steps(args.warmup)
+    # the source tables of the timed steps go to the device before the clock
+    # starts
+    sim.prepare(args.steps)
     torch.cuda.synchronize()
     # HIP events around the kernels of every 10th step (every step of a
     # short run): enough launches for a stable average, few enough that the
@@ -557,28 +611,34 @@ def main():
     h_ms, nh = eng.ctx.timing(0)
     e_ms, ne = eng.ctx.timing(1)
     f_ms, nf = eng.ctx.timing(3)
+    f2_ms, nf2 = eng.ctx.timing(5)
     eng.ctx.set_timing(False)
     cells = args.n**3
     value = cells * args.steps / el / 1e9
-    # average kernel time of one step: the fused H+E kernel when it ran for
-    # (nearly) every step, else the separate H and E kernels
-    if nf >= nh:
-        kern_ms, kname = f_ms, "k_yee_fused (H+E, one pass per step)"
+    # the kernel that advances (nearly) every step: the two-step pass (two
+    # launches, slab tiles and interior tiles), else the fused H+E kernel,
+    # else the separate H and E kernels
+    per_step_bytes = algorithmic_bytes(eng.plan)
+    if nf2 and nf2 >= nf:
+        kern_ms, kname, steps_per_launch = (
+            f2_ms, "k_yee_fused2_slab + k_yee_fused2_in (two steps per pass)", 2)
+    elif nf >= nh:
+        kern_ms, kname, steps_per_launch = f_ms, "k_yee_fused (H+E, one pass per step)", 1
     else:
-        kern_ms, kname = h_ms + e_ms, "k_yee_h + k_yee_e (one step)"
-
-    alg_bytes = algorithmic_bytes(eng.plan)
+        kern_ms, kname, steps_per_launch = h_ms + e_ms, "k_yee_h + k_yee_e (one step)", 1
+    alg_bytes = per_step_bytes * steps_per_launch
     achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
-    launch = {k: tuned.get(k) for k in ("vec", "ty", "kchunk", "order")}
+    launch = {k: tuned.get(k) for k in ("steps", "vec", "ty", "kchunk", "order")}
     wkey = f"voxel_3d_{args.n}_fp32_fp16"
-    traffic = measured_traffic(wkey, launch) if nf >= nh else None
+    traffic = measured_traffic(wkey, launch)
     finite = all(bool(torch.isfinite(sim.fields[c]).all()) for c in ("Ex", "Ey", "Ez"))
     ref = {c: sim.fields[c].clone() for c in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")}
-    print(f"[bench] {args.steps} steps in {el:.3f} s, fused {f_ms:.4f} ms x{nf}, "
-          f"H {h_ms:.4f} ms x{nh}, E {e_ms:.4f} ms x{ne}, "
+    print(f"[bench] {args.steps} steps in {el:.3f} s, two-step {f2_ms:.4f} ms x{nf2}, "
+          f"fused {f_ms:.4f} ms x{nf}, H {h_ms:.4f} ms x{nh}, E {e_ms:.4f} ms x{ne}, "
           f"finite={finite}", file=sys.stderr, flush=True)
     del sim, eng
     torch.cuda.empty_cache()
+    accuracy = None if args.no_fp64 else voxel_accuracy(args.n, tuned)
     fp64 = None if args.no_fp64 else voxel_fp64(args.n, args.steps, args.warmup, ref)
     del ref
     torch.cuda.empty_cache()
@@ -622,9 +682,11 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "alg_bytes_per_launch": alg_bytes,
+            "steps_per_launch": steps_per_launch,
             "kernel_ms": round(kern_ms, 5),
         },
         "cpu_baseline": cpu,
+        "voxel_accuracy": accuracy,
         "voxel_fp64": fp64,
         "sweep": sweep,
         "strip_2d": strip,

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define PM_ABI_VERSION 6
+#define PM_ABI_VERSION 7
 #define PM_SCRATCH_BYTES 4096
 
 /* error codes (< 0) */
@@ -196,6 +196,11 @@ typedef struct pm_plan {
   void* E2[3];
   void* H2[3];
   void* psi_h2[3][2];
+  /* Optional: a second E-side CPML memory set (same layout as psi_e).  With
+   * it (for every psi_e given), FP32 plans with Cb in FP32 or FP16, no
+   * dispersive boxes, padded rows of 128 or 256 cells and at most 64 x CPML
+   * layers can advance two steps per launch (pm_set_tuning fuse = 2). */
+  void* psi_e2[3][2];
 } pm_plan;
 
 typedef struct pm_ctx pm_ctx;
@@ -262,10 +267,15 @@ int pm_dft_flush(pm_ctx* ctx, int64_t step_first, int32_t rows);
  * and tile order (0: y-tiles fastest, 1: z-chunks fastest, each XCD taking a
  * contiguous range; 2: y-tiles fastest in hardware round-robin; 3: as 2 in
  * groups of 64 tiles of which each XCD takes 8 neighbours); fuse = 0 forces
- * the unfused kernels.  Defaults: 1, 4, 8, 0, 2.  Used by the autotuner. */
+ * the unfused kernels, fuse = 2 asks for two steps per launch (temporal
+ * blocking; ty must be 4 and vec 0; runs that sample monitors or add
+ * sources with a per-cell profile, and the one to three steps of a run that
+ * do not fill pairs of passes, take the one-step kernel).  Defaults: 1, 4,
+ * 8, 0, 2.  Used by the autotuner. */
 int pm_set_tuning(pm_ctx* ctx, int fuse, int ty, int kchunk, int vec, int order);
 
-/* 1 if pm_run will use the fused kernel with the current plan and tuning. */
+/* Steps per fused launch pm_run will use with the current plan and tuning
+ * for a run without monitor sampling: 0 (unfused kernels), 1 or 2. */
 int pm_fused_active(pm_ctx* ctx);
 
 /* Block until all work on the context's stream has finished. */
@@ -274,7 +284,7 @@ int pm_sync(pm_ctx* ctx);
 /* Average kernel time (ms) since timing was enabled, measured with HIP
  * events on the context stream around every launch of kernel class `which`
  * (0 = H update, 1 = E update, 2 = whole step, 3 = fused H+E step,
- * 4 = spectral flush of one sample block); enabled (and reset) by
+ * 4 = spectral flush of one sample block, 5 = two-step launch); enabled (and reset) by
  * pm_set_timing.  enable = 0 turns timing off; enable = s >= 1 times the
  * kernels of every s-th step of pm_run (steps whose index is a multiple of
  * s), so a long run can be timed with few events. */

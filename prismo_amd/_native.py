@@ -13,7 +13,7 @@ import os
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libpmfdtd.so"
-ABI_VERSION = 6
+ABI_VERSION = 7
 SCRATCH_BYTES = 4096
 
 PM_EX, PM_EY, PM_EZ, PM_HX, PM_HY, PM_HZ = range(6)
@@ -91,6 +91,7 @@ class pm_plan(C.Structure):
         ("nfreq_total", i32), ("pad4", i32),
         ("scratch", vp),
         ("E2", vp * 3), ("H2", vp * 3), ("psi_h2", (vp * 2) * 3),
+        ("psi_e2", (vp * 2) * 3),
     ]
 
 
@@ -274,13 +275,19 @@ class Context:
         sum rows of X as the samples of steps [step_first, +rows)."""
         self._check(self.lib.pm_dft_flush(self.h, step_first, rows), "pm_dft_flush")
 
-    def set_tuning(self, fuse: bool = True, ty: int = 4, kchunk: int = 16, vec: int = 0,
+    def set_tuning(self, fuse=True, ty: int = 4, kchunk: int = 16, vec: int = 0,
                    order: int = 0):
-        self._check(self.lib.pm_set_tuning(self.h, 1 if fuse else 0, ty, kchunk, vec, order),
-                    "pm_set_tuning")
+        """fuse: False/0 unfused, True/1 one step per launch, 2 two steps
+        per launch (ty 4, vec 0)."""
+        f = int(fuse) if not isinstance(fuse, bool) else (1 if fuse else 0)
+        self._check(self.lib.pm_set_tuning(self.h, f, ty, kchunk, vec, order), "pm_set_tuning")
 
     def fused_active(self) -> bool:
         return bool(self.lib.pm_fused_active(self.h))
+
+    def fused_steps(self) -> int:
+        """Steps per fused launch of a run without monitor sampling (0, 1, 2)."""
+        return int(self.lib.pm_fused_active(self.h))
 
     def sync(self):
         self._check(self.lib.pm_sync(self.h), "pm_sync")

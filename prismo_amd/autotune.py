@@ -1,8 +1,9 @@
 """Autotuner for the fused Yee step's launch shape.
 
 Times each candidate (cells per lane, rows per workgroup, z planes per
-workgroup, tile order) for a few steps with HIP events on the library's own
-stream, keeps the fastest, and restores the simulation state it started from:
+workgroup, tile order, and one or two steps per launch) for a few steps with
+HIP events on the library's own stream, keeps the fastest per step, and
+restores the simulation state it started from:
 fields, CPML memory and dispersive state are snapshotted on the device first,
 so tuning leaves a simulation exactly where it was.  Results are cached per
 (grid, precision, coefficient storage, CPML, ADE) in memory and, when
@@ -21,6 +22,12 @@ CANDIDATES = {
     "vec": (1, 2),
     "ty": (2, 3, 4),
     "kchunk": (8, 12, 16, 24),
+    "order": (2, 3),
+}
+# two steps per launch: rows per workgroup and cells per lane are fixed by
+# the kernel (4 rows, the padded row width), z planes and tile order vary
+CANDIDATES2 = {
+    "kchunk": (16, 24, 32, 48, 64),
     "order": (2, 3),
 }
 
@@ -74,27 +81,36 @@ def tune(eng, steps: int = 6, warmup: int = 2, candidates: dict | None = None,
     key = _key(eng)
     if key in _CACHE and not force:
         best = _CACHE[key]
-        eng.ctx.set_tuning(True, best["ty"], best["kchunk"], best["vec"], best["order"])
+        eng.ctx.set_tuning(best.get("steps", 1), best["ty"], best["kchunk"], best["vec"],
+                           best["order"])
         return dict(best, fused=True, cached=True)
     cand = candidates or CANDIDATES
+    shapes = [(1, vec, ty, kc, order) for vec, ty, kc, order in
+              product(cand["vec"], cand["ty"], cand["kchunk"], cand["order"])]
+    if candidates is None or "two_step" in candidates:
+        c2 = (candidates or {}).get("two_step", CANDIDATES2)
+        shapes += [(2, 0, 4, kc, order) for kc, order in product(c2["kchunk"], c2["order"])]
     saved = [t.clone() for t in _state_tensors(eng)]
     best = None
     try:
-        for vec, ty, kc, order in product(cand["vec"], cand["ty"], cand["kchunk"], cand["order"]):
+        for nstep, vec, ty, kc, order in shapes:
             try:
-                eng.ctx.set_tuning(True, ty, kc, vec, order)
-                if not eng.ctx.fused_active():
+                eng.ctx.set_tuning(nstep, ty, kc, vec, order)
+                if eng.ctx.fused_steps() != nstep:
                     continue
-                eng.ctx.run(0, warmup if warmup % 2 == 0 else warmup + 1, 0)
+                # whole pairs of passes (4 steps) keep the two-step path on
+                run = lambda n: eng.ctx.run(0, -(-n // 4) * 4, 0)
+                run(warmup)
                 eng.ctx.set_timing(True)
-                eng.ctx.run(0, steps if steps % 2 == 0 else steps + 1, 0)
-                ms, n = eng.ctx.timing(3)
+                run(steps)
+                ms, n = eng.ctx.timing(3 if nstep == 1 else 5)
                 eng.ctx.set_timing(False)
             except (ValueError, NotImplementedError):
                 eng.ctx.set_timing(False)
                 continue
-            if n and (best is None or ms < best["ms"]):
-                best = {"vec": vec, "ty": ty, "kchunk": kc, "order": order, "ms": ms}
+            if n and (best is None or ms / nstep < best["ms"]):
+                best = {"steps": nstep, "vec": vec, "ty": ty, "kchunk": kc, "order": order,
+                        "ms": ms / nstep, "ms_launch": ms}
     finally:
         for t, s in zip(_state_tensors(eng), saved):
             t.copy_(s)
@@ -102,7 +118,7 @@ def tune(eng, steps: int = 6, warmup: int = 2, candidates: dict | None = None,
     if best is None:
         eng.ctx.set_tuning(True, 4, 8, 0, 2)
         return {"fused": False}
-    eng.ctx.set_tuning(True, best["ty"], best["kchunk"], best["vec"], best["order"])
+    eng.ctx.set_tuning(best["steps"], best["ty"], best["kchunk"], best["vec"], best["order"])
     _CACHE[key] = best
     _save_file()
     return dict(best, fused=True, cached=False)

@@ -117,7 +117,7 @@ class StepEngine:
             cells = nx * ny * nz
             for a, n in enumerate((nx, ny, nz)):
                 S = cpml.thickness[2 * a] + cpml.thickness[2 * a + 1]
-                total += 6 * (S * cells // n) * esz + 6 * 256
+                total += 8 * (S * cells // n) * esz + 8 * 256
         self._arena = _torch().zeros((total,), dtype=_torch().uint8, device=self.device)
         self._arena_off = 0
 
@@ -367,15 +367,16 @@ class StepEngine:
         for a in range(3):
             S = p.pml[2 * a] + p.pml[2 * a + 1]
             shape = {0: (nz, ny, S), 1: (nz, S, nx), 2: (S, ny, nx)}[a]
-            if p.psi_h[a][0] and p.psi_h[a][1]:
-                pair = self._carve((2,) + shape, self.tdtype)
-                for q in range(2):
-                    self._keep.append(pair[q])
-                    p.psi_h2[a][q] = pair[q].data_ptr()
-            else:
-                for q in range(2):
-                    if p.psi_h[a][q]:
-                        p.psi_h2[a][q] = self._alloc(shape, self.tdtype).data_ptr()
+            for src, dst in ((p.psi_h, p.psi_h2), (p.psi_e, p.psi_e2)):
+                if src[a][0] and src[a][1]:
+                    pair = self._carve((2,) + shape, self.tdtype)
+                    for q in range(2):
+                        self._keep.append(pair[q])
+                        dst[a][q] = pair[q].data_ptr()
+                else:
+                    for q in range(2):
+                        if src[a][q]:
+                            dst[a][q] = self._alloc(shape, self.tdtype).data_ptr()
         self.fused = True
 
     def _fill_ade(self, regions):
@@ -589,7 +590,7 @@ class StepEngine:
         p = self.plan
         for a in range(3):
             for q in range(2):
-                for v in (p.psi_e[a][q], p.psi_h[a][q], p.psi_h2[a][q]):
+                for v in (p.psi_e[a][q], p.psi_h[a][q], p.psi_h2[a][q], p.psi_e2[a][q]):
                     if v:
                         ptrs.add(v)
         return [t for t in self._keep if t.data_ptr() in ptrs]

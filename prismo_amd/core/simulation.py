@@ -138,11 +138,9 @@ class Simulation:
     def step(self) -> None:
         self.run_steps(1)
 
-    def run_steps(self, steps: int, progress_callback=None, progress_interval: int = 10,
-                  _start=None) -> None:
-        if steps <= 0:
-            return
-        start = time_module.time() if _start is None else _start
+    def _io_plan(self, steps: int):
+        """Per-step tables of the next ``steps`` steps: times accumulated in
+        FP64 from the current time, device source terms, monitors."""
         eng = self.solver.updater.engine_for(self.fields)
         dt = self.dt
         times = []
@@ -169,8 +167,37 @@ class Simulation:
         T = 128 if esz == 4 else 512
         if any_td and ncols:
             T = int(max(1, min(T, steps, (256 << 20) // max(1, ncols * esz))))
+        return eng, times_arr, terms, py_sources, dev_mons, py_mons, any_td, T
+
+    def prepare(self, steps: int) -> None:
+        """Upload the source and monitor tables of the next ``steps`` steps
+        now, so that a following ``run_steps(steps)`` starts stepping at
+        once (it uses these tables as long as nothing was stepped, added or
+        prepared in between)."""
+        if steps <= 0:
+            return
+        plan = self._io_plan(steps)
+        eng, times_arr, terms, _, dev_mons, _, _, T = plan
+        eng.configure_io(terms, dev_mons, times_arr, step0=self.step_count, sample_T=T)
+        self._prepared = ((self.step_count, steps, len(self.sources), len(self.monitors),
+                           id(eng)), plan)
+
+    def run_steps(self, steps: int, progress_callback=None, progress_interval: int = 10,
+                  _start=None) -> None:
+        if steps <= 0:
+            return
+        start = time_module.time() if _start is None else _start
+        dt = self.dt
         step0 = self.step_count
-        eng.configure_io(terms, dev_mons, times_arr, step0=step0, sample_T=T)
+        prep = getattr(self, "_prepared", None)
+        self._prepared = None
+        eng = self.solver.updater.engine_for(self.fields)
+        key = (step0, steps, len(self.sources), len(self.monitors), id(eng))
+        if prep is not None and prep[0] == key:
+            eng, times_arr, terms, py_sources, dev_mons, py_mons, any_td, T = prep[1]
+        else:
+            eng, times_arr, terms, py_sources, dev_mons, py_mons, any_td, T = self._io_plan(steps)
+            eng.configure_io(terms, dev_mons, times_arr, step0=step0, sample_T=T)
         for m in dev_mons:
             if hasattr(m, "_note_dt"):
                 m._note_dt(dt)
@@ -179,7 +206,9 @@ class Simulation:
         n = step0
         end = step0 + steps
         while n < end:
-            seg = min(T - (n % T), end - n)
+            # time-domain samples come back one block of rows at a time; the
+            # spectra are summed by the library at block ends within any run
+            seg = min(T - (n % T), end - n) if any_td else end - n
             if per_step:
                 seg = 1
             if progress_callback is not None:

@@ -27,6 +27,14 @@ struct pm_ctx {
   bool fused_ok = false;
   int fuse = 1, tune_ty = 4, tune_kchunk = 8, tune_vec = 0, tune_order = 2;
   int parity = 0;
+  // two steps per launch: plan eligible, and which E-side CPML memory set
+  // holds the state (0 = plan.psi_e, 1 = plan.psi_e2)
+  bool fused2_ok = false;
+  // second stream for the slab tiles of a two-step pass (default priority)
+  hipStream_t s2 = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+  bool src_kind0 = true;  // every source has one value per step (kind 0)
+  int pe_parity = 0;
   KAux aux2{};  // source/sample view of the second buffer set
   std::vector<KAde> ade_host;  // staging of the ADE table upload
   std::vector<KSbox> sbox_host;  // staging of the box table uploads
@@ -37,7 +45,7 @@ struct pm_ctx {
   bool span_on = false;   // timing, and the current step is one of those
   std::vector<hipEvent_t> pool;
   size_t pool_used = 0;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> spans[5];
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> spans[6];
 };
 
 // Scratch of pm_configure_static: one 4 KiB copy on every device the
@@ -342,7 +350,7 @@ bool fused_shape(const pm_ctx* c, int* vec, int* w, int* ty) {
 }
 
 FusedBufs fused_bufs(const pm_plan& p, bool a_in) {
-  FusedBufs fb{};
+  FusedBufs fb{};  // pse_out stays NULL: the one-step kernel updates psi_e in place
   for (int a = 0; a < 3; ++a) {
     fb.Ei[a] = a_in ? p.E[a] : p.E2[a];
     fb.Hi[a] = a_in ? p.H[a] : p.H2[a];
@@ -383,6 +391,75 @@ int do_fused(pm_ctx* c) {
     return fail(c, PM_ENOTSUP, "fused step: no kernel for vec %d, w %d, ty %d", vec, w, ty);
   c->parity ^= 1;
   return launch_status(c, "fused step launch");
+}
+
+// Point the kernels' E-side CPML memory at the set that holds the state.
+void set_psi_e(pm_ctx* c) {
+  for (int a = 0; a < 3; ++a)
+    for (int q = 0; q < 2; ++q)
+      c->g.psi_e[a][q] = c->pe_parity ? c->plan.psi_e2[a][q] : c->plan.psi_e[a][q];
+}
+
+// Launch shape of the two-step kernel: the padded row decides the cells per
+// lane (two waves per row).
+bool fused2_shape(const pm_ctx* c, int* vec, int* w) {
+  const int rw = (int)c->plan.sy;
+  *w = 2;
+  *vec = rw / 128;
+  return (rw == 128 || rw == 256) && fused2_supported(*vec, *w, 4);
+}
+
+FusedBufs fused2_bufs(const pm_ctx* c) {
+  const pm_plan& p = c->plan;
+  FusedBufs fb = fused_bufs(p, c->parity == 0);
+  for (int a = 0; a < 3; ++a)
+    for (int q = 0; q < 2; ++q) fb.pse_out[a][q] = c->pe_parity ? p.psi_e[a][q] : p.psi_e2[a][q];
+  return fb;
+}
+
+// Two steps (n, n+1) in one launch from the current sets into the others,
+// with the sources of step n added inside the kernel.
+int do_fused2(pm_ctx* c, int64_t n, bool src) {
+  Span sp(c, 5);
+  const pm_plan& p = c->plan;
+  int vec, w;
+  fused2_shape(c, &vec, &w);
+  const FusedBufs fb = fused2_bufs(c);
+  const KAux& ax = c->parity ? c->aux2 : c->aux;
+  const int r = launch_yee_fused2_f32(c->g, p.coeff_kind, fb, vec, w, 4, c->tune_kchunk,
+                                      c->tune_order, src ? n - p.table_step0 : 0,
+                                      src ? ax.n_src : 0, ax.src_dev, c->stream, c->s2,
+                                      c->fork, c->join);
+  if (r == -2) return hip_check(c, hipGetLastError(), "two-step stream fork/join");
+  if (r != 0) return fail(c, PM_ENOTSUP, "two-step launch: no kernel for vec %d, w %d", vec, w);
+  c->parity ^= 1;
+  c->pe_parity ^= 1;
+  set_psi_e(c);
+  return launch_status(c, "two-step launch");
+}
+
+bool fused2_eligible(const pm_ctx* c) {
+  const pm_plan& p = c->plan;
+  if (!c->fused_ok || p.precision != PM_FP32 || p.n_ade || !fused2_grid_ok(c->g)) return false;
+  if (p.coeff_kind != PM_COEFF_CB && p.coeff_kind != PM_COEFF_CB_F16) return false;
+  int vec, w;
+  if (!fused2_shape(c, &vec, &w)) return false;
+  for (int a = 0; a < 3; ++a)
+    for (int q = 0; q < 2; ++q)
+      if (p.psi_e[a][q] && !p.psi_e2[a][q]) return false;
+  for (int pp = 0; pp < 2; ++pp) {
+    for (int pe = 0; pe < 2; ++pe) {
+      KGrid g = c->g;
+      FusedBufs fb = fused_bufs(p, pp == 0);
+      for (int a = 0; a < 3; ++a)
+        for (int q = 0; q < 2; ++q) {
+          g.psi_e[a][q] = pe ? p.psi_e2[a][q] : p.psi_e[a][q];
+          fb.pse_out[a][q] = pe ? p.psi_e[a][q] : p.psi_e2[a][q];
+        }
+      if (!fused_addressable(g, fb, 4)) return false;
+    }
+  }
+  return true;
 }
 
 bool fused_eligible(const pm_plan& p) {
@@ -446,12 +523,28 @@ int pm_create(int device, void* stream, pm_ctx** out) {
   pm_ctx* c = new pm_ctx();
   c->device = device;
   c->stream = (hipStream_t)stream;
+  // the two-step pass's second stream and its fork/join events; without
+  // them the pass runs its two launches one after the other
+  if (hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess) {
+    (void)hipGetLastError();
+    if (c->fork) (void)hipEventDestroy(c->fork);
+    if (c->join) (void)hipEventDestroy(c->join);
+    if (c->s2) (void)hipStreamDestroy(c->s2);
+    c->s2 = nullptr;
+    c->fork = c->join = nullptr;
+  }
   *out = c;
   return 0;
 }
 
 int pm_destroy(pm_ctx* c) {
   if (!c) return 0;
+  if (c->s2) (void)hipStreamSynchronize(c->s2);
+  if (c->fork) (void)hipEventDestroy(c->fork);
+  if (c->join) (void)hipEventDestroy(c->join);
+  if (c->s2) (void)hipStreamDestroy(c->s2);
   for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
   delete c;
   return 0;
@@ -622,6 +715,12 @@ int pm_configure(pm_ctx* c, const pm_plan* plan) {
                 fused_addressable(c->g, fused_bufs(p, true), p.precision == PM_FP64 ? 8 : 4) &&
                 fused_addressable(c->g, fused_bufs(p, false), p.precision == PM_FP64 ? 8 : 4);
   c->parity = 0;
+  c->pe_parity = 0;
+  set_psi_e(c);
+  c->fused2_ok = fused2_eligible(c);
+  c->src_kind0 = true;
+  for (int s = 0; s < x.n_src; ++s)
+    if (x.src[s].kind != 0) c->src_kind0 = false;
   c->pend_n = 0;
   c->configured = true;
   return 0;
@@ -718,26 +817,52 @@ int pm_run(pm_ctx* c, int64_t step0, int32_t nsteps, uint32_t flags) {
   }
   int fvec = 0, fw = 0, fty = 0;
   const bool fused = c->fused_ok && c->fuse && fused_shape(c, &fvec, &fw, &fty);
-  for (int32_t s = 0; s < nsteps; ++s) {
+  // two steps per launch for the steps that fill pairs of passes (an even
+  // number of passes, so the E-side CPML memory ends in its first set); the
+  // one to three steps left over go first: an odd one unfused in place, a
+  // pair through the one-step kernel (an even number of ping-pong launches,
+  // so the fields end in the first set too)
+  const bool two = fused && c->fuse == 2 && c->fused2_ok && !smp && nsteps >= 4 &&
+                   (!src || c->src_kind0);
+  const int32_t head = two ? nsteps % 4 : 0;
+  for (int32_t s = 0; s < nsteps;) {
     const int64_t n = step0 + s;
     c->span_on = c->timing && n % c->timing_stride == 0;
-    Span whole(c, 2);
     int r;
-    // fused runs keep an even number of ping-pong steps: an odd run does its
-    // first step in place, so the call ends with the state in the first set
-    if (fused && !(s == 0 && (nsteps & 1))) {
-      r = do_fused(c);
-    } else {
-      r = do_h(c);
-      if (!r) r = do_e(c);
+    if (two && s >= head) {
+      {
+        Span whole(c, 2);
+        r = do_fused2(c, n, src);
+        if (r) return r;
+        if (src) {
+          launch_sources(c->parity ? c->aux2 : c->aux, n + 1, c->stream);
+          if ((r = launch_status(c, "source launch"))) return r;
+        }
+      }
+      s += 2;
+      continue;
     }
-    if (r) return r;
-    const KAux& ax = c->parity ? c->aux2 : c->aux;
-    if (src) {
-      launch_sources(ax, n, c->stream);
-      if ((r = launch_status(c, "source launch"))) return r;
+    {
+      Span whole(c, 2);
+      // fused runs keep an even number of ping-pong steps: an odd run does
+      // its first step in place, so the call ends with the state in the
+      // first set
+      const int32_t one = two ? head : nsteps;
+      if (fused && !(s == 0 && (one & 1))) {
+        r = do_fused(c);
+      } else {
+        r = do_h(c);
+        if (!r) r = do_e(c);
+      }
+      if (r) return r;
+      const KAux& ax = c->parity ? c->aux2 : c->aux;
+      if (src) {
+        launch_sources(ax, n, c->stream);
+        if ((r = launch_status(c, "source launch"))) return r;
+      }
+      if (smp && (r = sample_step(c, n, dft))) return r;
     }
-    if (smp && (r = sample_step(c, n, dft))) return r;
+    ++s;
   }
   c->span_on = c->timing;
   return 0;
@@ -779,7 +904,10 @@ int pm_set_tuning(pm_ctx* c, int fuse, int ty, int kchunk, int vec, int order) {
   if (vec != 0 && vec != 1 && vec != 2 && vec != 4)
     return fail(c, PM_EINVAL, "vec must be 0 (auto), 1, 2 or 4");
   if (order < 0 || order > 3) return fail(c, PM_EINVAL, "order must be 0, 1, 2 or 3");
-  c->fuse = fuse != 0;
+  if (fuse < 0 || fuse > 2) return fail(c, PM_EINVAL, "fuse must be 0, 1 or 2");
+  if (fuse == 2 && (ty != 4 || vec != 0))
+    return fail(c, PM_EINVAL, "two steps per launch take ty 4 and vec 0 (auto)");
+  c->fuse = fuse;
   c->tune_ty = ty;
   c->tune_kchunk = kchunk;
   c->tune_vec = vec;
@@ -791,7 +919,8 @@ int pm_fused_active(pm_ctx* c) {
   if (!c || !c->configured) return 0;
   int vec = 0, w = 0, ty = 0;
   if (!c->fused_ok) return 0;
-  return (c->fuse && fused_shape(c, &vec, &w, &ty)) ? 1 : 0;
+  if (!(c->fuse && fused_shape(c, &vec, &w, &ty))) return 0;
+  return (c->fuse == 2 && c->fused2_ok) ? 2 : 1;
 }
 
 int pm_sync(pm_ctx* c) {
@@ -811,7 +940,7 @@ int pm_set_timing(pm_ctx* c, int enable) {
 }
 
 int pm_get_timing(pm_ctx* c, int which, double* avg_ms, int64_t* count) {
-  if (!c || which < 0 || which > 4 || !avg_ms || !count) return PM_EINVAL;
+  if (!c || which < 0 || which > 5 || !avg_ms || !count) return PM_EINVAL;
   int r = hip_check(c, hipStreamSynchronize(c->stream), "stream sync");
   if (r) return r;
   double tot = 0;

@@ -43,6 +43,7 @@ struct KFz {
   uint32_t cb, ccs; // Cb of Ex; bytes between the three Cb arrays
   uint32_t ca[3], da[3], db[3];
   uint32_t pe[3], phi[3], pho[3];  // CPML memory per axis: q = 0 then q = 1
+  uint32_t peo[3];                 // E-side memory written (two-step pass; else = pe)
   uint32_t ps[3];                  // bytes of one CPML memory array of axis a
   uint32_t prof[3];                // per axis: be ce ike bh ch ikh, n_a each
   uint32_t pn[3];                  // bytes of one profile array of axis a
@@ -894,6 +895,7 @@ inline bool make_kfz_impl(const KGrid& g, const FusedBufs& fb, int esz, KFz* out
     add(g.psi_e[a][0]);
     add(fb.psh_in[a][0]);
     add(fb.psh_out[a][0]);
+    add(fb.pse_out[a][0]);
   }
   if (nr == 0) PM_KFZ_FAIL("no coefficient arrays");
   uintptr_t lo = ~(uintptr_t)0;
@@ -921,7 +923,7 @@ inline bool make_kfz_impl(const KGrid& g, const FusedBufs& fb, int esz, KFz* out
       // no CPML on this axis: the kernel's unconditional CPML-memory loads
       // (slab position 0, values unused) read the Cb arrays instead, which
       // are larger than any position they can form
-      z.pe[a] = z.phi[a] = z.pho[a] = z.prof[a] = z.cb;
+      z.pe[a] = z.phi[a] = z.pho[a] = z.prof[a] = z.peo[a] = z.cb;
       z.ps[a] = z.pn[a] = 0;
       continue;
     }
@@ -937,6 +939,12 @@ inline bool make_kfz_impl(const KGrid& g, const FusedBufs& fb, int esz, KFz* out
     if (!off(g.psi_e[a][0], &z.pe[a]) || !off(fb.psh_in[a][0], &z.phi[a]) ||
         !off(fb.psh_out[a][0], &z.pho[a]))
       PM_KFZ_FAIL("CPML memory beyond 4 GiB");
+    z.peo[a] = z.pe[a];
+    if (fb.pse_out[a][0]) {
+      if (!fb.pse_out[a][1] || u(fb.pse_out[a][1]) - u(fb.pse_out[a][0]) != ps)
+        PM_KFZ_FAIL("E-side CPML memory (written set) not packed");
+      if (!off(fb.pse_out[a][0], &z.peo[a])) PM_KFZ_FAIL("CPML memory beyond 4 GiB");
+    }
     const uintptr_t pn = (uintptr_t)n[a] * (uintptr_t)esz;
     const void* pr6[6] = {g.pml_be[a], g.pml_ce[a], g.pml_ike[a],
                           g.pml_bh[a], g.pml_ch[a], g.pml_ikh[a]};

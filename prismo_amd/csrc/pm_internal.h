@@ -163,6 +163,8 @@ struct FusedBufs {
   void* Ho[3];
   const void* psh_in[3][2];
   void* psh_out[3][2];
+  // two-step pass only: E-side CPML memory written (read: KGrid::psi_e)
+  void* pse_out[3][2];
 };
 
 // Launchers (defined in the .hip files).
@@ -176,6 +178,17 @@ int launch_yee_fused_f32(const KGrid& g, int ck, const FusedBufs& fb, int vec, i
 int launch_yee_fused_f64(const KGrid& g, int ck, const FusedBufs& fb, int vec, int w, int ty,
                          int kchunk, int order, hipStream_t s);
 bool fused_supported(int precision, int vec, int w, int ty);
+// Two Yee steps per launch (fused2_kernel.h), FP32 fields, Cb in FP32 or
+// FP16, no dispersive boxes; the sources of the first step (table row
+// src_row, n_src entries of the device table src) are added inside.
+// The slab tiles go to stream s2 (forked from s with event fork, joined back
+// with event join) so they run beside the interior tiles; s2 = NULL keeps
+// both launches on s.
+int launch_yee_fused2_f32(const KGrid& g, int ck, const FusedBufs& fb, int vec, int w, int ty,
+                          int kchunk, int order, int64_t src_row, int n_src, const KSrc* src,
+                          hipStream_t s, hipStream_t s2, hipEvent_t fork, hipEvent_t join);
+bool fused2_supported(int vec, int w, int ty);
+bool fused2_grid_ok(const KGrid& g);  // x CPML layers fit the kernel's LDS slots
 // The plan's arrays are laid out as the fused kernel addresses them
 // (packed field sets, Cb, CPML memory pairs and profiles within 4 GiB).
 bool fused_addressable(const KGrid& g, const FusedBufs& fb, int esz);
