@@ -42,15 +42,21 @@ components from the FP32/FP16 run at the end (FP16 Cb quantisation
 included).  ``--no-strip`` / ``--no-waveguide`` / ``--no-sweep`` /
 ``--no-fp64`` skip those parts.
 
-``--surrogate`` runs only the surrogate-screened coupler search (4096
-FDTD training spectra, a 4-member BF16 MLP ensemble, 65536 screened
-designs, FDTD check of the top 64 and the uncertain ones) and prints its
-own JSON line.
+``surrogate``: the surrogate-screened coupler search (4096 FDTD training
+spectra, a 4-member BF16 MLP ensemble, 65536 screened designs, FDTD check
+of the top 64 and the uncertain ones): FDTD-confirmed designs per second end
+to end, the |S21| / |S31| error of the surrogate against the FDTD and the
+time of each stage; ``--no-surrogate`` skips it, ``--surrogate`` runs only
+it and prints its own JSON line.
+
+``voxel_accuracy``: the FP32/FP16 voxel fields after 1000 steps against an
+FP64 run with the same (FP16-rounded) Cb and against FP64 with exact Cb.
 
 ``--copy-probe`` runs only a 256 MiB device copy (the counter calibration
 kernel); ``--batch-probe`` only 8 eager steps of the 256-coupler batch (for
-counter runs on the batch kernel); ``--launch vec,ty,kchunk,order`` fixes
-the fused launch shape instead of autotuning.
+counter runs on the batch kernel); ``--launch vec,ty,kchunk,order[,steps]``
+fixes the fused launch shape (and one or two steps per launch) instead of
+autotuning.
 """
 
 from __future__ import annotations
@@ -284,7 +290,9 @@ def sweep_bench(n_designs=256, steps=4000, seq_sample=12, group=64, first=16, ro
     from prismo_amd.optimization.sweep import BatchedRunner, ParameterSweep
 
     designs = W.coupler_designs(n_designs)
-    kw = {"chunk": 8, "sample_T": 128}
+    # rows per tile and launch shape from the autotuner (the untimed warm-up
+    # below tunes each group size once; the timed run uses the cached picks)
+    kw = {"chunk": "auto", "sample_T": 128}
     runner = BatchedRunner(build=lambda p: W.coupler_sim(p, backend=W.host_backend()),
                            analyze=lambda sim, p: W.coupler_analyze(sim), steps=steps,
                            dtype=np.float32, coefficient_storage="fp16", max_batch=group,
@@ -314,6 +322,7 @@ def sweep_bench(n_designs=256, steps=4000, seq_sample=12, group=64, first=16, ro
     sims = [W.coupler_sim(p, backend=W.host_backend()) for p in designs]
     eng = BatchEngine(sims, dtype=np.float32, coefficient_storage="fp16", **kw)
     eng.run_steps(roof_steps, graph=True, timing=True)
+    tuning = eng.tuning
     g_ms, g_n = eng.timing(2)
     e_ms, e_n = eng.timing(0)
     f_ms, f_n = eng.timing(1)
@@ -364,7 +373,7 @@ def sweep_bench(n_designs=256, steps=4000, seq_sample=12, group=64, first=16, ro
                      "achieved": round(alg / (step_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(alg / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                      "alg_bytes_per_step": alg, "step_ms": round(step_ms, 5),
-                     "steps_timed": roof_steps},
+                     "steps_timed": roof_steps, "launch": tuning},
         "precision": "fp32 fields, fp16 Cb, fp32 compensated spectra; baseline fp64",
     }
 
@@ -534,6 +543,7 @@ def main():
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--surrogate", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--sweep-only", action="store_true")
     ap.add_argument("--no-strip", action="store_true")
     ap.add_argument("--no-waveguide", action="store_true")
@@ -575,11 +585,7 @@ def main():
         print(json.dumps({"waveguide_3d": waveguide_bench()}), flush=True)
         return
     if args.surrogate:
-        from prismo_amd.surrogate import loop
-
-        res = loop.run(n_train=4096, n_screen=65536, top_k=64, epochs=300,
-                       log=lambda m: print(m, file=sys.stderr, flush=True))
-        print(json.dumps({"surrogate": res}), flush=True)
+        print(json.dumps({"surrogate": surrogate_bench()}), flush=True)
         return
     sim = voxel_3d(dtype=np.float32, coefficient_storage="fp16", n=args.n)
     eng = sim.solver.updater.engine_for(sim.fields)
@@ -647,6 +653,7 @@ def main():
         sweep["cpu_baseline"] = cpu_sweep
         sweep["speedup_vs_cpu_pool"] = round(sweep["designs_per_s"] / cpu_sweep["designs_per_s"], 1)
     strip = None if args.no_strip else strip_bench()
+    surrogate = None if args.no_surrogate else surrogate_bench()
     waveguide = None if args.no_waveguide else waveguide_bench()
     cpu = None
     if not args.no_cpu_baseline:
@@ -691,8 +698,21 @@ def main():
         "sweep": sweep,
         "strip_2d": strip,
         "waveguide_3d": waveguide,
+        "surrogate": surrogate,
     }
     print(json.dumps(line), flush=True)
+
+
+def surrogate_bench() -> dict:
+    """The surrogate loop end to end: 4096 coupler designs (gap, length,
+    width, cladding) through the batched FDTD for training spectra, a
+    4-member BF16 MLP ensemble trained on the matrix cores, 65536 designs
+    screened, the top 64 plus the uncertain ones re-run by the batched FDTD
+    as judge.  ``accepted_per_s`` counts only designs the FDTD confirmed."""
+    from prismo_amd.surrogate import loop
+
+    return loop.run(n_train=4096, n_screen=65536, top_k=64, epochs=300,
+                    log=lambda m: print(m, file=sys.stderr, flush=True))
 
 
 if __name__ == "__main__":

@@ -106,6 +106,22 @@ int64_t pm_batch_steps(pm_batch* b);
 
 int pm_batch_sync(pm_batch* b);
 
+/* Per design (n must equal the plan's n_design), waits for the stream and
+ * writes bit 0 = some Ez value was Inf or NaN since pm_batch_configure.  A
+ * design that diverges cannot disturb the others (their arithmetic never
+ * reads it), so a caller keeps their results and reports this one failed.
+ * No counterpart in the reference, which runs each design in its own
+ * Simulation (src/prismo/optimization/sweep.py:190-195 catches per design). */
+int pm_batch_design_status(pm_batch* b, int32_t* out, int32_t n);
+
+/* Launch shape of the step kernel: waves (64 cells wide each) per
+ * workgroup, 1, 2, 4 (default) or 8, and the workgroup order (1, default:
+ * each run of 64 workgroups is grouped by XCD so neighbouring strips share
+ * an L2; 0: launch order).  Rows per tile are the plan's `chunk`.  Results
+ * do not depend on any of them.  Tunes what the reference's backends take as
+ * fixed launch constants (src/prismo/backends/cuda_kernels.py:230-265). */
+int pm_batch_set_tuning(pm_batch* b, int waves, int xcd_order);
+
 /* Average time (ms) per step kernel (which 0) or per flush (1) since enabled. */
 int pm_batch_set_timing(pm_batch* b, int enable);
 int pm_batch_get_timing(pm_batch* b, int which, double* avg_ms, int64_t* count);

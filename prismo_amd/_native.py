@@ -173,6 +173,8 @@ BATCH_SIGNATURES = {
     "pm_batch_parity": ([vp], C.c_int),
     "pm_batch_steps": ([vp], i64),
     "pm_batch_sync": ([vp], C.c_int),
+    "pm_batch_design_status": ([vp, C.POINTER(i32), C.c_int32], C.c_int),
+    "pm_batch_set_tuning": ([vp, C.c_int, C.c_int], C.c_int),
     "pm_batch_set_timing": ([vp, C.c_int], C.c_int),
     "pm_batch_get_timing": ([vp, C.c_int, C.POINTER(f64), C.POINTER(i64)], C.c_int),
 }
@@ -364,6 +366,19 @@ class BatchContext:
 
     def sync(self):
         self._check(self.lib.pm_batch_sync(self.h), "pm_batch_sync")
+
+    def set_tuning(self, waves: int = 4, xcd: int = 1):
+        """Waves per workgroup (1, 2, 4, 8) and XCD-grouped order (0/1)."""
+        self._check(self.lib.pm_batch_set_tuning(self.h, int(waves), int(xcd)),
+                    "pm_batch_set_tuning")
+
+    def design_status(self, n: int):
+        """Per design, bit 0: an Ez value was Inf or NaN (waits for the stream)."""
+        import numpy as np
+
+        out = (i32 * n)()
+        self._check(self.lib.pm_batch_design_status(self.h, out, n), "pm_batch_design_status")
+        return np.array(out[:], dtype=np.int32)
 
     def set_timing(self, on: bool):
         self._check(self.lib.pm_batch_set_timing(self.h, 1 if on else 0), "pm_batch_set_timing")

@@ -78,13 +78,20 @@ class SParameterAnalyzer:
             self.s_matrix[:, port_index, excitation_port] = (np.asarray(power_forward) / inc) ** 0.5
 
     def add_mode_data(self, port_index: int, excitation_port: int, mode_coefficients: dict) -> None:
+        """The reference's definitions (sparameters.py:88-122): reflection
+        S_ii = backward / forward, transmission S_ij = forward / backward of
+        the coefficients at port i.  Added: with an ``'incident'`` key (the
+        exciting port's forward coefficient), transmission is forward /
+        incident instead, the usual mode-expansion S_ij; without it the
+        reference's quotient is kept."""
         fwd = mode_coefficients.get("forward", np.zeros_like(self.frequencies))
         bwd = mode_coefficients.get("backward", np.zeros_like(self.frequencies))
         if port_index == excitation_port:
             self.s_matrix[:, port_index, excitation_port] = bwd / fwd
+        elif "incident" in mode_coefficients:
+            self.s_matrix[:, port_index, excitation_port] = fwd / mode_coefficients["incident"]
         else:
-            inc = mode_coefficients.get("incident", fwd)
-            self.s_matrix[:, port_index, excitation_port] = fwd / inc
+            self.s_matrix[:, port_index, excitation_port] = fwd / bwd
 
     def get_s_parameter(self, i: int, j: int) -> np.ndarray:
         return self.s_matrix[:, i, j]

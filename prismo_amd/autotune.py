@@ -1,4 +1,5 @@
-"""Autotuner for the fused Yee step's launch shape.
+"""Autotuners for the fused Yee step's launch shape and for the batched
+2-D step (``tune_batch``).
 
 Times each candidate (cells per lane, rows per workgroup, z planes per
 workgroup, tile order, and one or two steps per launch) for a few steps with
@@ -29,6 +30,15 @@ CANDIDATES = {
 CANDIDATES2 = {
     "kchunk": (16, 24, 32, 48, 64),
     "order": (2, 3),
+}
+
+
+# batched 2-D step: rows per tile (the plan's chunk), waves per workgroup,
+# XCD-grouped workgroup order
+BATCH_CANDIDATES = {
+    "chunk": (4, 8, 16, 32),
+    "waves": (2, 4, 8),
+    "xcd": (0, 1),
 }
 
 
@@ -122,3 +132,56 @@ def tune(eng, steps: int = 6, warmup: int = 2, candidates: dict | None = None,
     _CACHE[key] = best
     _save_file()
     return dict(best, fused=True, cached=False)
+
+
+def _batch_key(eng) -> str:
+    # designs per batch rounded up to a power of two: a sweep's groups of
+    # one size share a pick whatever the design sizes in them
+    n = 1 << max(0, (len(eng.sims) - 1).bit_length())
+    return json.dumps(["batch", int(eng.precision), int(eng.coeff_kind), n])
+
+
+def tune_batch(eng, steps: int = 16, candidates: dict | None = None,
+               force: bool = False) -> dict:
+    """Pick and apply the fastest launch shape of the batched step for the
+    configured, not yet run ``BatchEngine`` ``eng``: each candidate (rows per
+    tile, waves per workgroup, workgroup order) runs ``steps`` eager steps
+    timed with HIP events on the engine's stream; then the engine's state is
+    cleared back to the zeros a run starts from and it is re-planned with
+    the pick.  Results are bitwise the same under every shape (each cell's
+    arithmetic does not depend on the tiling); cached per (precision,
+    coefficient storage, designs per batch) like ``tune``.
+
+    Returns {"chunk", "waves", "xcd", "ms", "cached"}."""
+    _load_file()
+    key = _batch_key(eng)
+    cand = candidates or BATCH_CANDIDATES
+    if key in _CACHE and not force:
+        best = dict(_CACHE[key], cached=True)
+    else:
+        n = max(1, min(steps, eng.table_len))
+        best = None
+        try:
+            for i, chunk in enumerate(cand["chunk"]):
+                for waves, xcd in product(cand["waves"], cand["xcd"]):
+                    eng.reconfigure(chunk)
+                    eng.ctx.set_tuning(waves, xcd)
+                    if i == 0 and best is None:
+                        eng.ctx.run(min(2, n))    # kernel code loaded, untimed
+                        eng.reconfigure(chunk)
+                    eng.ctx.set_timing(True)
+                    eng.ctx.run(n)
+                    ms, cnt = eng.ctx.timing(0)
+                    eng.ctx.set_timing(False)
+                    if cnt and (best is None or ms < best["ms"]):
+                        best = {"chunk": chunk, "waves": waves, "xcd": xcd, "ms": ms}
+        finally:
+            eng.ctx.set_timing(False)
+            eng.ctx.sync()
+            eng._zero_state()
+        _CACHE[key] = best
+        _save_file()
+        best = dict(best, cached=False)
+    eng.reconfigure(best["chunk"])
+    eng.ctx.set_tuning(best["waves"], best["xcd"])
+    return best

@@ -38,7 +38,7 @@ def _torch():
 
 class BatchEngine:
     def __init__(self, sims, dtype=np.float32, coefficient_storage: str = "auto",
-                 chunk: int = 8, sample_T: int = 128, device: int = 0):
+                 chunk=8, sample_T: int = 128, device: int = 0):
         torch = _torch()
         if not sims:
             raise ValueError("empty batch")
@@ -66,7 +66,11 @@ class BatchEngine:
             if s.dt != self.dt or tuple(s.grid.spacing) != self.d:
                 raise NotImplementedError("every design must share dx, dy and dt")
         self.sample_T = int(sample_T)
-        self.chunk = int(chunk)
+        # rows per tile; "auto": the autotuner picks them with the launch
+        # shape when the run starts (prismo_amd.autotune.tune_batch)
+        self.auto_tune = chunk == "auto"
+        self.chunk = 8 if self.auto_tune else int(chunk)
+        self.tuning = None
         self._keep: list = []
         self.step_count = 0
         self._cb_q: list = []   # per design: (stored Cb, FP16 scale or None)
@@ -139,17 +143,9 @@ class BatchEngine:
         # monitors -> sample lines
         nl = 0
         mons = []
-        for m in s.monitors:
-            if not isinstance(m, FieldMonitor) or m.time_domain or not m.frequencies:
-                raise NotImplementedError("batched designs take frequency-domain FieldMonitors")
-            spec = m.device_spec()
+        for m, lines in self._monitor_lines(s):
             cols = {}
-            for comp, box, shape in spec["boxes"]:
-                (i0, i1), (j0, j1), _ = box
-                if comp not in _COMP_LINE or i1 - i0 != 1:
-                    raise NotImplementedError("batched monitors are Ez/Hx/Hy lines along y")
-                if nl >= N.BATCH_MAX_LINES:
-                    raise NotImplementedError("at most 8 sample lines per design")
+            for comp, i0, j0, j1 in lines:
                 L = des.line[nl]
                 L.comp, L.i, L.j0, L.j1, L.col = _COMP_LINE[comp], i0, j0, j1, col
                 cols[comp] = (col, col + (j1 - j0))
@@ -159,7 +155,58 @@ class BatchEngine:
         des.n_lines = nl
         return col, mons
 
-    def _sources(self, s, times):
+    @staticmethod
+    def _monitor_lines(s) -> list:
+        """Per monitor of design s, its sample lines (comp, i, j0, j1)."""
+        out = []
+        nl = 0
+        for m in s.monitors:
+            if not isinstance(m, FieldMonitor) or m.time_domain or not m.frequencies:
+                raise NotImplementedError("batched designs take frequency-domain FieldMonitors")
+            lines = []
+            for comp, box, shape in m.device_spec()["boxes"]:
+                (i0, i1), (j0, j1), _ = box
+                if comp not in _COMP_LINE or i1 - i0 != 1:
+                    raise NotImplementedError("batched monitors are Ez/Hx/Hy lines along y")
+                if nl >= N.BATCH_MAX_LINES:
+                    raise NotImplementedError("at most 8 sample lines per design")
+                lines.append((comp, i0, j0, j1))
+                nl += 1
+            out.append((m, lines))
+        return out
+
+    @classmethod
+    def design_key(cls, s, nsteps: int) -> tuple:
+        """For a Simulation a batch can take, what every design of one batch
+        must share (dt, dx and dy, the monitors' frequency list, the source
+        waveforms over ``nsteps`` steps); raises ``NotImplementedError`` with
+        the reason for one it cannot take.  A sweep groups designs by this
+        key and runs the others one at a time."""
+        if not s.grid.is_2d:
+            raise NotImplementedError("batched stepping is 2-D")
+        up = s.solver.updater
+        if up.scheme != "yee":
+            raise NotImplementedError("batched stepping needs scheme='yee'")
+        if not is_lossless_nonmagnetic(up.mu_rel, up.sigma_e, up.sigma_m):
+            raise NotImplementedError("batched stepping needs lossless, non-magnetic media")
+        freqs = None
+        for m, _ in cls._monitor_lines(s):
+            f = tuple(float(v) for v in m.frequencies)
+            if freqs is None:
+                freqs = f
+            elif f != freqs:
+                raise NotImplementedError("every monitor of a batch shares one frequency list")
+        times = []
+        t = 0.0
+        for _ in range(nsteps):
+            t += s.dt
+            times.append(t)
+        e, h = cls._sources(s, np.array(times))
+        return (float(s.dt), tuple(float(v) for v in s.grid.spacing), freqs,
+                None if e is None else e[3].tobytes(), None if h is None else h[3].tobytes())
+
+    @staticmethod
+    def _sources(s, times):
         e = h = None
         for src in s.sources:
             terms = src.device_terms(times)
@@ -185,6 +232,7 @@ class BatchEngine:
         sizes = [arr[d].ny * arr[d].sy for d in range(n)]
         per = 2 * len(self._FIELD_ARRAYS) + 2
         arena = self._alloc((per * sum(sizes),))
+        self._arena = arena
         cb_all = torch.from_numpy(np.concatenate([c.ravel() for c in self._host_cb]))
         cbd = cb_all.to(self.device, torch.float16 if self.coeff_kind == N.COEFF_CB_F16
                         else self.tdtype)
@@ -281,6 +329,7 @@ class BatchEngine:
         p.table_step0, p.table_len = 0, max(nsteps, 1)
         T = self.sample_T
         X = self._alloc((T, self.ncols))
+        self._X = X
         p.X, p.ncols, p.sample_T = X.data_ptr(), self.ncols, T
         nf = len(self.freqs)
         p.nfreq = nf
@@ -304,6 +353,13 @@ class BatchEngine:
             p.W_re, p.W_im = Wre.data_ptr(), Wim.data_ptr()
         p.chunk = self.chunk
         wsb = self.ctx.workspace_bytes(p)
+        if self.auto_tune:
+            from ..autotune import BATCH_CANDIDATES
+
+            for c in BATCH_CANDIDATES["chunk"]:
+                p.chunk = c
+                wsb = max(wsb, self.ctx.workspace_bytes(p))
+            p.chunk = self.chunk
         ws = self._alloc((wsb,), torch.uint8)
         p.work, p.work_bytes = ws.data_ptr(), wsb
         self.plan = p
@@ -315,6 +371,24 @@ class BatchEngine:
         self.ctx.configure(p, keep=[arr])
         self.table_len = max(nsteps, 1)
         self._built = True
+
+    def _zero_state(self):
+        """Fields, CPML memory, sample rows and spectra back to zero (the
+        state a configured run starts from), on the engine's stream."""
+        torch = _torch()
+        with torch.cuda.stream(self.stream):
+            self._arena.zero_()
+            self._X.zero_()
+            for a in (self.acc or []):
+                if a is not None:
+                    a.zero_()
+
+    def reconfigure(self, chunk: int):
+        """Re-plan with ``chunk`` rows per tile; the step counter restarts."""
+        self.chunk = int(chunk)
+        self.plan.chunk = self.chunk
+        self.stream.wait_stream(_torch().cuda.current_stream(self.device))
+        self.ctx.configure(self.plan, keep=[self._arr])
 
     # ---------------------------------------------------------------- running
     def run_steps(self, nsteps: int, graph: bool = True, timing: bool = False) -> None:
@@ -332,6 +406,10 @@ class BatchEngine:
         if self._built:
             raise RuntimeError("a BatchEngine runs once; build a new one for another run")
         self._build(nsteps)
+        if self.auto_tune:
+            from ..autotune import tune_batch
+
+            self.tuning = tune_batch(self)
         T = self.sample_T
         self.ctx.set_timing(timing)
         self.ctx.run(nsteps, graph=graph and T % 2 == 0)
@@ -346,6 +424,9 @@ class BatchEngine:
             raise RuntimeError("finish() without start()")
         self._queued = None
         self.ctx.sync()
+        # per design: did its fields or spectra leave the finite numbers?
+        st = self.ctx.design_status(len(self.sims))
+        self.diverged = [bool(v & 1) for v in st]
         self.step_count = nsteps
         t = 0.0
         for _ in range(nsteps):
@@ -364,8 +445,11 @@ class BatchEngine:
             I = (im.double() + cim.double()).cpu().numpy()
         else:
             R, I = re.cpu().numpy(), im.cpu().numpy()
-        for mons in self.mon:
+        for d, mons in enumerate(self.mon):
             for m, cols in mons:
+                if not all(np.isfinite(R[:, c0:c1]).all() and np.isfinite(I[:, c0:c1]).all()
+                           for c0, c1 in cols.values()):
+                    self.diverged[d] = True
                 m._ingest_spectra({c: R[:, c0:c1] + 1j * I[:, c0:c1] for c, (c0, c1) in cols.items()})
 
     def fields(self, d: int) -> dict:

@@ -33,8 +33,8 @@
 namespace pm {
 namespace {
 
-constexpr int kStrip = 64;   // cells per wave
-constexpr int kWaves = 4;    // waves per workgroup
+constexpr int kStrip = 64;    // cells per wave
+constexpr int kMaxWaves = 8;  // waves per workgroup: 1, 2, 4 (default) or 8
 
 struct KWork {
   int d, x0, r0, r1;
@@ -46,6 +46,9 @@ struct KBatch {
   int n_work;
   int ncols, sample_T;
   int64_t* clock;
+  int32_t* status;  // per design: bit 0 set once an Ez value was not finite
+  int waves;        // waves per workgroup (the launch's blockDim.x / 64)
+  int xcd;          // 1: XCD-grouped workgroup order, 0: launch order
   double idx, idy;
   const double* tab_e;
   const double* tab_h;
@@ -92,6 +95,19 @@ __device__ __forceinline__ T lane_value(T v, int l) {
   }
 }
 
+// Bit 31 set exactly for Inf and NaN: the exponent field plus one carries
+// into the sign position only when the field is all ones.  Integer VALU
+// work, OR-ed into one register per lane (a compare would hold a lane mask
+// in scalar registers across the row loop), and no floating-point mode can
+// fold it away.
+template <typename T>
+__device__ __forceinline__ uint32_t nonfinite_bit(T v) {
+  if constexpr (sizeof(T) == 4)
+    return (__builtin_bit_cast(uint32_t, v) & 0x7f800000u) + 0x00800000u;
+  else
+    return ((uint32_t)(__builtin_bit_cast(uint64_t, v) >> 32) & 0x7ff00000u) + 0x00100000u;
+}
+
 template <typename T, int CK>
 struct RowE {
   using CbT = typename std::conditional<CK == PM_COEFF_CB_F16, unsigned short, T>::type;
@@ -100,8 +116,9 @@ struct RowE {
 };
 
 template <typename T, int CK>
-__global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_t step_arg,
-                                                          int use_clock, int s_rel, int par) {
+__global__ __launch_bounds__(64 * kMaxWaves) void k_batch_tm(const KBatch k, int64_t step_arg,
+                                                             int use_clock, int s_rel,
+                                                             int par) {
   const int lane = threadIdx.x & 63;
   // the wave index is uniform: make that visible so the tile and design
   // descriptors are scalar loads
@@ -110,8 +127,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
   // XCD x takes work groups 8x .. 8x+7, so neighbouring strips (which read
   // the same cache lines at their edges) share one L2
   const int b = blockIdx.x, nb = gridDim.x;
-  const int wg = b < (nb & ~63) ? (b & ~63) + (b & 7) * 8 + ((b >> 3) & 7) : b;
-  const int item = wg * kWaves + wv;
+  const int wg = k.xcd && b < (nb & ~63) ? (b & ~63) + (b & 7) * 8 + ((b >> 3) & 7) : b;
+  const int item = wg * k.waves + wv;
   if (item >= k.n_work) return;  // whole wave
   const KWork w = k.work[item];
   const pm_batch_design& D = k.des[w.d];
@@ -229,6 +246,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
   const int elast = max(jend - 1, r0);  // last E row the tile updates (or r0)
   T ezp = (r0 >= 1 && inx) ? ldg<T>(EzA, (uint32_t)(r0 - 1) * syu + (uint32_t)i) : (T)0;
   T hxp = 0, hyp = 0, hyp_r = 0;
+  uint32_t bad = 0;  // bit 31: this lane computed an Inf or NaN Ez
 
   // one row: H(n+1/2) at row j from `cur`, E(n+1) at row j-1 from `ecur`;
   // first loads row j+2 into `nxt` and the E row j+1 into `enxt` (two rows
@@ -325,6 +343,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
       if (valid) {
         ezn = ezp + cb * (tx - ty);
         if (srcE && jj >= se0 && jj < se1) ezn += se;
+        bad |= nonfinite_bit(ezn);
       }
       if (own) {
         stg<T>(EzB, oe, ezn);
@@ -360,6 +379,9 @@ __global__ __launch_bounds__(64 * kWaves) void k_batch_tm(const KBatch k, int64_
     body(j, h2, h1, e2, e1);
     if (++j > jend) break;
   }
+  // a diverged design is flagged once per tile (rare: a vector atomic from
+  // the lanes that saw it)
+  if (bad >> 31) atomicOr(k.status + w.d, 1);
 }
 
 __global__ void k_clock_set(int64_t* clock, int64_t v) {
@@ -389,6 +411,8 @@ struct pm_batch {
   int64_t pend0 = 0;  // first step of the pending sample block
   int pend_n = 0;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
+  int waves = 4;  // launch shape of the step kernel (pm_batch_set_tuning)
+  int xcd = 1;
   bool timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> spans[3];
   std::string err;
@@ -430,12 +454,14 @@ struct BSpan {
 
 int64_t align256(int64_t v) { return (v + 255) & ~(int64_t)255; }
 
-void layout(const pm_batch_plan& p, const std::vector<KWork>* work, int64_t* des_off,
-            int64_t* work_off, int64_t* total, int64_t n_work) {
-  *des_off = 256;  // the clock sits at offset 0
+// Workspace: the step clock at offset 0, then the design table, the tile
+// list and one status word per design.
+void layout(const pm_batch_plan& p, int64_t* des_off, int64_t* work_off, int64_t* status_off,
+            int64_t* total, int64_t n_work) {
+  *des_off = 256;
   *work_off = align256(*des_off + (int64_t)sizeof(pm_batch_design) * p.n_design);
-  *total = align256(*work_off + (int64_t)sizeof(KWork) * n_work);
-  (void)work;
+  *status_off = align256(*work_off + (int64_t)sizeof(KWork) * n_work);
+  *total = align256(*status_off + (int64_t)sizeof(int32_t) * p.n_design);
 }
 
 void build_work(const pm_batch_plan& p, std::vector<KWork>& work) {
@@ -509,12 +535,13 @@ int validate(pm_batch* b, const pm_batch_plan& p) {
 
 template <typename T>
 int launch_step(pm_batch* b, int64_t step_arg, int use_clock, int s_rel, int par) {
-  const int nb = ((int)b->work.size() + kWaves - 1) / kWaves;
+  const int wv = b->kb.waves;
+  const int nb = ((int)b->work.size() + wv - 1) / wv;
   if (b->plan.coeff_kind == PM_COEFF_CB_F16)
-    hipLaunchKernelGGL((k_batch_tm<T, PM_COEFF_CB_F16>), dim3(nb), dim3(64 * kWaves), 0,
+    hipLaunchKernelGGL((k_batch_tm<T, PM_COEFF_CB_F16>), dim3(nb), dim3(64 * wv), 0,
                        b->stream, b->kb, step_arg, use_clock, s_rel, par);
   else
-    hipLaunchKernelGGL((k_batch_tm<T, PM_COEFF_CB>), dim3(nb), dim3(64 * kWaves), 0,
+    hipLaunchKernelGGL((k_batch_tm<T, PM_COEFF_CB>), dim3(nb), dim3(64 * wv), 0,
                        b->stream, b->kb, step_arg, use_clock, s_rel, par);
   return bhip(b, hipGetLastError(), "batch step launch");
 }
@@ -607,8 +634,8 @@ int pm_batch_workspace_bytes(const pm_batch_plan* p, int64_t* bytes) {
   if (!p || !bytes || p->n_design < 1 || !p->design || p->chunk < 2) return PM_EINVAL;
   std::vector<KWork> w;
   build_work(*p, w);
-  int64_t a, c, t;
-  layout(*p, &w, &a, &c, &t, (int64_t)w.size());
+  int64_t a, c, st, t;
+  layout(*p, &a, &c, &st, &t, (int64_t)w.size());
   *bytes = t;
   return 0;
 }
@@ -622,8 +649,8 @@ int pm_batch_configure(pm_batch* b, const pm_batch_plan* p) {
   if (r) return r;
   std::vector<KWork> work;
   build_work(*p, work);
-  int64_t des_off, work_off, total;
-  layout(*p, &work, &des_off, &work_off, &total, (int64_t)work.size());
+  int64_t des_off, work_off, status_off, total;
+  layout(*p, &des_off, &work_off, &status_off, &total, (int64_t)work.size());
   if (!p->work || p->work_bytes < total)
     return bfail(b, PM_EINVAL, "workspace of %lld bytes required", (long long)total);
   b->plan = *p;
@@ -635,7 +662,10 @@ int pm_batch_configure(pm_batch* b, const pm_batch_plan* p) {
   k.clock = (int64_t*)ws;
   k.des = (const pm_batch_design*)(ws + des_off);
   k.work = (const KWork*)(ws + work_off);
+  k.status = (int32_t*)(ws + status_off);
   k.n_work = (int)b->work.size();
+  k.waves = b->waves;
+  k.xcd = b->xcd;
   k.ncols = p->ncols;
   k.sample_T = p->sample_T;
   k.idx = 1.0 / p->dx;
@@ -652,6 +682,12 @@ int pm_batch_configure(pm_batch* b, const pm_batch_plan* p) {
   if (!r)
     r = bhip(b, hipMemcpyAsync((void*)k.work, b->work.data(), sizeof(KWork) * b->work.size(),
                                hipMemcpyHostToDevice, b->stream), "tile list upload");
+  // status words start clear (an upload, not a memset: nothing here is
+  // captured, but the same holds for every buffer a graph reads)
+  const std::vector<int32_t> zeros((size_t)p->n_design, 0);
+  if (!r)
+    r = bhip(b, hipMemcpyAsync(k.status, zeros.data(), sizeof(int32_t) * zeros.size(),
+                               hipMemcpyHostToDevice, b->stream), "status upload");
   // the uploads read pageable host memory owned by this object: finish them
   if (!r) r = bhip(b, hipStreamSynchronize(b->stream), "upload sync");
   if (r) return r;
@@ -742,6 +778,34 @@ int pm_batch_sync(pm_batch* b) {
   if (!b) return PM_EINVAL;
   (void)hipSetDevice(b->device);
   return bhip(b, hipStreamSynchronize(b->stream), "sync");
+}
+
+int pm_batch_set_tuning(pm_batch* b, int waves, int xcd_order) {
+  if (!b) return PM_EINVAL;
+  if (waves != 1 && waves != 2 && waves != 4 && waves != 8)
+    return bfail(b, PM_EINVAL, "waves per workgroup must be 1, 2, 4 or 8");
+  if (xcd_order != 0 && xcd_order != 1) return bfail(b, PM_EINVAL, "xcd_order must be 0 or 1");
+  (void)hipSetDevice(b->device);
+  // a captured graph holds the old launch shape
+  drop_graphs(b);
+  b->waves = b->kb.waves = waves;
+  b->xcd = b->kb.xcd = xcd_order;
+  return 0;
+}
+
+int pm_batch_design_status(pm_batch* b, int32_t* out, int32_t n) {
+  if (!b || !out) return PM_EINVAL;
+  if (!b->configured) return bfail(b, PM_ENOPLAN, "pm_batch_design_status before configure");
+  if (n != b->plan.n_design)
+    return bfail(b, PM_EINVAL, "status of %d designs asked, the plan has %d", n,
+                 b->plan.n_design);
+  (void)hipSetDevice(b->device);
+  int r = bhip(b, hipStreamSynchronize(b->stream), "status sync");
+  if (!r)
+    r = bhip(b, hipMemcpyAsync(out, b->kb.status, sizeof(int32_t) * (size_t)n,
+                               hipMemcpyDeviceToHost, b->stream), "status download");
+  if (!r) r = bhip(b, hipStreamSynchronize(b->stream), "status sync");
+  return r;
 }
 
 int pm_batch_set_timing(pm_batch* b, int enable) {

@@ -14,12 +14,17 @@ step over every design of a group of ``max_batch``), then calls
 ``analyze(sim, params)`` per design.  Groups are pipelined: group g+1 is
 built and queued on its own stream while group g runs, and group g is
 analysed while group g+1 runs.  A design whose build or analysis raises is
-reported with status 'error' and does not stop the others.
+reported with status 'error' and does not stop the others; so is one whose
+fields leave the finite numbers (the kernel flags each design that computes
+an Inf or NaN).  A design the batch cannot take (3-D, lossy, a time-domain
+monitor, another source waveform or frequency list than the group's) runs
+alone through its own ``Simulation.run_steps``.
 """
 
 from __future__ import annotations
 
 import json
+from collections import Counter
 from concurrent.futures import ProcessPoolExecutor
 from dataclasses import dataclass, field
 from pathlib import Path
@@ -124,7 +129,12 @@ class ParameterSweep:
             except Exception as e:
                 fail([k for k, _ in group], e)
                 return
-            for k, sim in group:
+            for d, (k, sim) in enumerate(group):
+                if eng.diverged[d]:
+                    # the reference would report NaN results as a success;
+                    # a run that left the finite numbers is a failed design
+                    fail([k], "fields diverged (Inf or NaN) during the batched run")
+                    continue
                 try:
                     r = dict(br.analyze(sim, combos[k]))
                     r["parameters"] = combos[k]
@@ -133,18 +143,40 @@ class ParameterSweep:
                 except Exception as e:
                     fail([k], e)
 
+        def run_alone(k, sim):
+            try:
+                sim.run_steps(br.steps)
+                r = dict(br.analyze(sim, combos[k]))
+                r["parameters"] = combos[k]
+                r["status"] = "success"
+                results[k] = r
+            except Exception as e:
+                fail([k], e)
+
         pending = None  # (engine, group) queued on the GPU, not yet analysed
         starts = [0]
         first = br.first_batch if br.first_batch else br.max_batch
         starts += list(range(min(first, len(combos)), len(combos), br.max_batch))
         for gi, g0 in enumerate(starts):
             g1 = starts[gi + 1] if gi + 1 < len(starts) else len(combos)
-            group = []
+            keyed, singles = [], []
             for k in range(g0, g1):
                 try:
-                    group.append((k, br.build(combos[k])))
+                    sim = br.build(combos[k])
                 except Exception as e:
                     fail([k], e)
+                    continue
+                try:
+                    keyed.append((k, sim, BatchEngine.design_key(sim, br.steps)))
+                except NotImplementedError:
+                    singles.append((k, sim))
+            # the group runs the designs sharing the most common key; a
+            # design a batch cannot take, or one of another key, runs alone
+            group = []
+            if keyed:
+                main = Counter(key for _, _, key in keyed).most_common(1)[0][0]
+                group = [(k, sim) for k, sim, key in keyed if key == main]
+                singles += [(k, sim) for k, sim, key in keyed if key != main]
             eng = None
             if group:
                 try:
@@ -157,6 +189,8 @@ class ParameterSweep:
             if pending is not None:
                 complete(*pending)
             pending = (eng, group) if eng is not None else None
+            for k, sim in sorted(singles):
+                run_alone(k, sim)
         if pending is not None:
             complete(*pending)
         return results

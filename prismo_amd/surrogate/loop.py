@@ -87,7 +87,7 @@ def fdtd(designs, steps=W.COUPLER_STEPS, max_batch=64, dtype=np.float32, storage
     runner = BatchedRunner(build=lambda p: W.coupler_sim(p, backend=W.host_backend()),
                            analyze=lambda sim, p: W.coupler_analyze(sim), steps=steps,
                            dtype=dtype, coefficient_storage=storage, max_batch=max_batch,
-                           engine_kw={"chunk": 8}, first_batch=min(16, max_batch))
+                           engine_kw={"chunk": "auto"}, first_batch=min(16, max_batch))
     res = ParameterSweep.from_samples(designs, batched=runner).run()
     bad = [r for r in res if r["status"] != "success"]
     if bad:

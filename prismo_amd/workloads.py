@@ -119,8 +119,10 @@ def host_backend():
 
 
 def coupler_sim(p, dtype=np.float32, backend=None, coefficient_storage="auto",
-                freqs=COUPLER_FREQS):
-    """Build (not run) the Simulation of one coupler design."""
+                freqs=COUPLER_FREQS, straight=False):
+    """Build (not run) the Simulation of one coupler design; ``straight``
+    leaves out the second strip (the normalisation run's straight guide on
+    the same grid, with the same source and ports)."""
     import prismo_amd as pa
 
     t = COUPLER_PML
@@ -134,8 +136,9 @@ def coupler_sim(p, dtype=np.float32, backend=None, coefficient_storage="auto",
     y2 = ymid + (g + w) / 2
     eps = np.full((nx, ny, 1), p["clad_eps"])
     eps[:, np.abs(y - y1) <= w / 2, 0] = 12.1
-    sel2 = np.abs(y - y2) <= w / 2
-    eps[np.ix_(x >= 1.5e-6, sel2, [0])] = 12.1
+    if not straight:
+        sel2 = np.abs(y - y2) <= w / 2
+        eps[np.ix_(x >= 1.5e-6, sel2, [0])] = 12.1
     sim = pa.Simulation(size=(nxp * D * SHRINK, nyp * D * SHRINK, 0.0), resolution=1 / D,
                         pml_layers=t, courant_factor=0.99, scheme="yee", dtype=dtype,
                         backend=backend, pml=pa.PMLParams(thickness=t),
@@ -158,10 +161,26 @@ def coupler_sim(p, dtype=np.float32, backend=None, coefficient_storage="auto",
     return sim
 
 
-def coupler_analyze(sim, params=None) -> dict:
+def coupler_norm(straight_sim) -> dict:
+    """Port powers of the normalisation run: the straight guide of
+    ``coupler_sim(p, straight=True)`` after it ran.  ``p_out`` (through plus
+    cross port) is the power that reaches the output plane's two windows
+    when nothing couples; ``coupler_analyze(..., norm=)`` divides by it."""
+    from .analysis.sparameters import port_power
+
+    ports = straight_sim.ports
+    pin, pt, pc = (port_power(ports[k], "x", +1) for k in ("in", "thru", "cross"))
+    return {"p_in": pin, "p_thru": pt, "p_cross": pc, "p_out": pt + pc}
+
+
+def coupler_analyze(sim, params=None, norm=None) -> dict:
     """|S21| (through) and |S31| (cross) from the port fluxes, normalised to
-    the input port's forward power; t21/t31 add the phase of each port's
-    summed Ez relative to the input port's."""
+    the input port's forward power (the reference's incident power,
+    sparameters.py:47-86) or, with ``norm`` from ``coupler_norm`` of the
+    design's straight guide, to the power that guide brings to the output
+    plane (what the source launches but the guide does not carry, and the
+    DFT's cut of a long pulse, then cancel); t21/t31 add the phase of each
+    port's summed Ez relative to the input port's."""
     from .analysis.sparameters import SParameterAnalyzer, port_power
 
     ports = sim.ports
@@ -169,7 +188,7 @@ def coupler_analyze(sim, params=None) -> dict:
     sa = SParameterAnalyzer(3, freqs)
     p_in = port_power(ports["in"], "x", +1)
     zero = np.zeros_like(p_in)
-    sa.add_port_data(0, 0, p_in, zero)
+    sa.add_port_data(0, 0, p_in if norm is None else norm["p_out"], zero)
     sa.add_port_data(1, 0, np.maximum(port_power(ports["thru"], "x", +1), 0.0), zero)
     sa.add_port_data(2, 0, np.maximum(port_power(ports["cross"], "x", +1), 0.0), zero)
     s21 = np.abs(sa.get_s_parameter(1, 0))

@@ -22,7 +22,7 @@ SIZES = [(70, 40), (130, 25), (45, 90), (52, 33)]
 FREQS = [187e12, 193e12, 200e12, 205e12]
 
 
-def _design(nxp, nyp, t, dtype, seed):
+def _design(nxp, nyp, t, dtype, seed, poison=False):
     import prismo_amd as pa
 
     rng = np.random.default_rng(seed)
@@ -34,6 +34,8 @@ def _design(nxp, nyp, t, dtype, seed):
     yc = ny // 2 + int(rng.integers(-3, 4))
     eps[:, yc - 5:yc + 6] = 12.1
     eps *= rng.uniform(1.0, 1.05, size=eps.shape)
+    if poison:
+        eps[nx // 2, ny // 2, 0] = np.nan
     sim = pa.Simulation(size=(nxp * D * s, nyp * D * s, 0.0), resolution=1 / D, pml_layers=t,
                         courant_factor=0.99, scheme="yee", pml=pa.PMLParams(t),
                         material_arrays={"eps_rel": eps}, dtype=dtype)
@@ -125,3 +127,50 @@ def test_batch_graph_equals_eager(mi355x):
         for c in ("Ez", "Hx", "Hy"):
             assert np.array_equal(fa[d][c], fb[d][c]), (d, c)
         assert np.array_equal(sa[d], sb[d]), d
+
+
+def _spectra(built):
+    return [np.stack([b[0].monitors[0].get_frequency_data(c, fr) for c in ("Ez", "Hy")
+                      for fr in FREQS]) for b in built]
+
+
+def test_batch_launch_shapes_and_autotune_are_bitwise(mi355x):
+    """Every launch shape (rows per tile, waves per workgroup, workgroup
+    order) gives the same bits, and so does the autotuner's pick; the tuner
+    leaves the engine at the zero state a run starts from."""
+    from prismo_amd import autotune
+    from prismo_amd.core.batch import BatchEngine
+
+    ref_eng, ref = _run(np.float32, "fp16", 130, True, chunk=8)
+    ref_f = [ref_eng.fields(d) for d in range(len(SIZES))]
+    ref_s = _spectra(ref)
+    runs = []
+    for chunk, waves, xcd in ((4, 8, 0), (16, 2, 1), (32, 1, 0)):
+        built = [_design(nxp, nyp, 8, np.float32, 100 + k) for k, (nxp, nyp) in enumerate(SIZES)]
+        eng = BatchEngine([b[0] for b in built], dtype=np.float32, coefficient_storage="fp16",
+                          chunk=chunk, sample_T=32)
+        eng.ctx.set_tuning(waves, xcd)
+        eng.run_steps(130, graph=True)
+        runs.append((eng, built))
+    autotune._CACHE.clear()
+    built = [_design(nxp, nyp, 8, np.float32, 100 + k) for k, (nxp, nyp) in enumerate(SIZES)]
+    eng = BatchEngine([b[0] for b in built], dtype=np.float32, coefficient_storage="fp16",
+                      chunk="auto", sample_T=32)
+    eng.run_steps(130, graph=True)
+    print("batch tuning:", eng.tuning)
+    assert eng.tuning and not eng.tuning["cached"]
+    assert eng.tuning["chunk"] in autotune.BATCH_CANDIDATES["chunk"]
+    runs.append((eng, built))
+    for eng, built in runs:
+        s = _spectra(built)
+        for d in range(len(SIZES)):
+            f = eng.fields(d)
+            for c in ("Ez", "Hx", "Hy"):
+                assert np.array_equal(f[c], ref_f[d][c]), (d, c)
+            assert np.array_equal(s[d], ref_s[d]), d
+    # a second engine of the same batch size takes the cached pick
+    built = [_design(nxp, nyp, 8, np.float32, 100 + k) for k, (nxp, nyp) in enumerate(SIZES)]
+    eng2 = BatchEngine([b[0] for b in built], dtype=np.float32, coefficient_storage="fp16",
+                       chunk="auto", sample_T=32)
+    eng2.run_steps(10, graph=False)
+    assert eng2.tuning["cached"]

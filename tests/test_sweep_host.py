@@ -7,6 +7,21 @@ from prismo_amd.analysis.sparameters import (SParameterAnalyzer, compute_group_d
 from prismo_amd.optimization.sweep import BatchedRunner, ParameterSweep, SweepParameter
 
 
+class _FakeBase:
+    """What the sweep needs of BatchEngine beyond start/finish: one group key
+    for every design, and no design diverged."""
+
+    class _Never:
+        def __getitem__(self, d):
+            return False
+
+    diverged = _Never()
+
+    @staticmethod
+    def design_key(sim, nsteps):
+        return 0
+
+
 def test_combinations_are_the_meshgrid_in_order():
     sw = ParameterSweep([SweepParameter("a", [1, 2]), SweepParameter("b", [10, 20, 30])],
                         lambda p: {"m": p["a"] * p["b"]})
@@ -35,7 +50,7 @@ def test_batched_build_errors_do_not_stop_the_sweep(monkeypatch):
 
     ran = {}
 
-    class FakeEngine:
+    class FakeEngine(_FakeBase):
         def __init__(self, sims, **kw):
             ran["n"] = len(sims)
 
@@ -68,7 +83,7 @@ def test_batched_groups_are_pipelined(monkeypatch):
 
     log = []
 
-    class FakeEngine:
+    class FakeEngine(_FakeBase):
         def __init__(self, sims, **kw):
             self.ids = [s["a"] for s in sims]
 
@@ -132,7 +147,7 @@ def test_batched_first_group_size(monkeypatch):
 
     groups = []
 
-    class FakeEngine:
+    class FakeEngine(_FakeBase):
         def __init__(self, sims, **kw):
             groups.append([s["a"] for s in sims])
 
@@ -149,3 +164,93 @@ def test_batched_first_group_size(monkeypatch):
                               first_batch=1))
     assert all(r["status"] == "success" for r in sw.run())
     assert groups == [[0], [1, 2, 3], [4, 5, 6]]
+
+
+def test_batched_diverged_design_is_an_error_alone(monkeypatch):
+    """A design the kernel flags as diverged (Inf/NaN) reports 'error' and is
+    not analysed; the other designs of its group are."""
+    import prismo_amd.core.batch as B
+
+    analysed = []
+
+    class FakeEngine(_FakeBase):
+        def __init__(self, sims, **kw):
+            self.ids = [s["a"] for s in sims]
+
+        def start(self, n, graph=True):
+            pass
+
+        def finish(self):
+            self.diverged = [a == 2 for a in self.ids]
+
+    monkeypatch.setattr(B, "BatchEngine", FakeEngine)
+    sw = ParameterSweep.from_samples(
+        [{"a": k} for k in range(4)],
+        batched=BatchedRunner(build=dict, analyze=lambda s, p: analysed.append(p["a"]) or {},
+                              steps=1))
+    res = sw.run()
+    assert [r["status"] for r in res] == ["success", "success", "error", "success"]
+    assert "diverged" in res[2]["error"]
+    assert analysed == [0, 1, 3]
+
+
+def test_unbatchable_design_runs_alone(monkeypatch):
+    """A design the batch cannot take (design_key raises) or of another key
+    runs through its own Simulation; its group still runs batched, and a
+    single that fails is the only error."""
+    import prismo_amd.core.batch as B
+
+    groups, alone = [], []
+
+    class FakeEngine(_FakeBase):
+        def __init__(self, sims, **kw):
+            groups.append([s.a for s in sims])
+
+        def start(self, n, graph=True):
+            pass
+
+        def finish(self):
+            pass
+
+        @staticmethod
+        def design_key(sim, nsteps):
+            if sim.a in (1, 5):
+                raise NotImplementedError("lossy media")
+            return "other" if sim.a == 3 else "main"
+
+    class Sim:
+        def __init__(self, p):
+            self.a = p["a"]
+
+        def run_steps(self, n):
+            if self.a == 5:
+                raise RuntimeError("single failed")
+            alone.append((self.a, n))
+
+    monkeypatch.setattr(B, "BatchEngine", FakeEngine)
+    sw = ParameterSweep.from_samples(
+        [{"a": k} for k in range(6)],
+        batched=BatchedRunner(build=Sim, analyze=lambda s, p: {"a": s.a}, steps=9))
+    res = sw.run()
+    assert groups == [[0, 2, 4]]
+    assert alone == [(1, 9), (3, 9)]
+    assert [r["status"] for r in res] == ["success"] * 5 + ["error"]
+    assert [r.get("a") for r in res[:5]] == [0, 1, 2, 3, 4]
+    assert "single failed" in res[5]["error"]
+
+
+def test_mode_data_keeps_the_reference_quotients():
+    """add_mode_data: S_ii = backward/forward; S_ij = forward/backward as the
+    reference defines it (sparameters.py:113-122) unless an 'incident'
+    coefficient is given, then forward/incident."""
+    f = np.array([1e14, 2e14])
+    sa = SParameterAnalyzer(2, f)
+    fwd = np.array([2.0 + 1.0j, 4.0])
+    bwd = np.array([0.5, 1.0 - 1.0j])
+    sa.add_mode_data(0, 0, {"forward": fwd, "backward": bwd})
+    assert np.allclose(sa.get_s_parameter(0, 0), bwd / fwd)
+    sa.add_mode_data(1, 0, {"forward": fwd, "backward": bwd})
+    assert np.allclose(sa.get_s_parameter(1, 0), fwd / bwd)
+    inc = np.array([8.0, 8.0j])
+    sa.add_mode_data(1, 0, {"forward": fwd, "backward": bwd, "incident": inc})
+    assert np.allclose(sa.get_s_parameter(1, 0), fwd / inc)
