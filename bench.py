@@ -52,7 +52,9 @@ it and prints its own JSON line.
 ``voxel_accuracy``: the FP32/FP16 voxel fields after 1000 steps against an
 FP64 run with the same (FP16-rounded) Cb and against FP64 with exact Cb.
 
-``--copy-probe`` runs only a 256 MiB device copy (the counter calibration
+``--waveguide-only [--waveguide-steps N]`` and ``--fp64-only`` run only
+those legs (short runs for counter collection).  ``--copy-probe`` runs only a
+256 MiB device copy (the counter calibration
 kernel); ``--batch-probe`` only 8 eager steps of the 256-coupler batch (for
 counter runs on the batch kernel); ``--launch vec,ty,kchunk,order[,steps]``
 fixes the fused launch shape (and one or two steps per launch) instead of
@@ -116,7 +118,7 @@ def voxel_fp64(n, steps, warmup, ref):
                for a in range(3))
     alg = 120 * cells + 64 * slab
     num = den = 0.0
-    for c in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
+    for c in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz") if ref is not None else ():
         f = sim.fields[c].double()
         num += float(((f - ref[c].double()) ** 2).sum())
         den += float((f ** 2).sum())
@@ -447,17 +449,27 @@ def strip_bench(cpu_steps=60):
 
     out = {}
     for scheme in ("compat", "yee"):
-        sim, mon, steps = W.strip_2d(scheme=scheme)
-        sim.run_steps(10)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        sim.run_steps(steps)
-        torch.cuda.synchronize()
-        el = time.perf_counter() - t0
-        nx, ny, _ = sim.grid.dimensions
-        out[scheme] = {"gcells_per_s": round(nx * ny * steps / el / 1e9, 4),
-                       "us_per_step": round(el / steps * 1e6, 2), "steps": steps}
-        del sim
+        res = {}
+        for graph in (True, False):
+            sim, mon, steps = W.strip_2d(scheme=scheme)
+            sim.graph = graph
+            # warm-up: one 128-step block (kernels loaded, with PM_RUN_GRAPH
+            # the block's graph captured)
+            sim.run_steps(128)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            sim.run_steps(steps)
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+            nx, ny, _ = sim.grid.dimensions
+            if graph:
+                res["us_per_step_graph_replay"] = round(el / steps * 1e6, 2)
+            else:
+                res.update({"gcells_per_s": round(nx * ny * steps / el / 1e9, 4),
+                            "us_per_step": round(el / steps * 1e6, 2), "steps": steps,
+                            "loop": "direct launches (the default)"})
+            del sim
+        out[scheme] = res
     sim, mon, _ = W.strip_2d(scheme="compat")
     nx, ny, nz = sim.grid.dimensions
     eps = W._strip_eps()["eps_rel"]
@@ -548,6 +560,8 @@ def main():
     ap.add_argument("--no-strip", action="store_true")
     ap.add_argument("--no-waveguide", action="store_true")
     ap.add_argument("--waveguide-only", action="store_true")
+    ap.add_argument("--waveguide-steps", type=int, default=None)
+    ap.add_argument("--fp64-only", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only; runs on several GPUs are not supported "
@@ -582,7 +596,11 @@ def main():
         print(json.dumps({"sweep": sweep}), flush=True)
         return
     if args.waveguide_only:
-        print(json.dumps({"waveguide_3d": waveguide_bench()}), flush=True)
+        print(json.dumps({"waveguide_3d": waveguide_bench(args.waveguide_steps)}), flush=True)
+        return
+    if args.fp64_only:
+        print(json.dumps({"voxel_fp64": voxel_fp64(args.n, args.steps, args.warmup, None)}),
+              flush=True)
         return
     if args.surrogate:
         print(json.dumps({"surrogate": surrogate_bench()}), flush=True)

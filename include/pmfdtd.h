@@ -241,7 +241,19 @@ int pm_update_e(pm_ctx* ctx);
 #define PM_RUN_SOURCES 1u
 #define PM_RUN_SAMPLE  2u
 #define PM_RUN_DFT     4u
-#define PM_RUN_GRAPH   8u   /* replay the step loop as captured HIP graphs */
+/* PM_RUN_GRAPH: whole blocks of G steps (G = sample_T with PM_RUN_SAMPLE,
+ * else 32; with sampling a block starts at a step that is a multiple of
+ * sample_T) run as replays of a HIP graph captured on the context's second
+ * stream the first time (one per starting buffer set and flag combination,
+ * kept until the next pm_configure / pm_set_tuning), ordered after the
+ * context stream's work and before its next; the other steps of the run are
+ * launched directly.  Each replay reads its first step from a device step
+ * clock in the plan's scratch (offset 512) for the source and phasor tables.
+ * The results are bitwise those of the same run without the flag.  Two steps
+ * per launch (pm_set_tuning fuse 2) are not used in a graph.  PM_EINVAL when
+ * G is odd.  Replaces the reference's Python step loop
+ * (src/prismo/core/simulation.py:147-164). */
+#define PM_RUN_GRAPH   8u
 /* Take no step: sample step step0 (nsteps must be 1) from the fields as
  * they are, with PM_RUN_SAMPLE / PM_RUN_DFT as above.  For callers that add
  * terms of their own after pm_run's step and sources (src/prismo/core/
@@ -284,7 +296,8 @@ int pm_sync(pm_ctx* ctx);
 /* Average kernel time (ms) since timing was enabled, measured with HIP
  * events on the context stream around every launch of kernel class `which`
  * (0 = H update, 1 = E update, 2 = whole step, 3 = fused H+E step,
- * 4 = spectral flush of one sample block, 5 = two-step launch); enabled (and reset) by
+ * 4 = spectral flush of one sample block, 5 = two-step launch, 6 = one graph
+ * replay of a PM_RUN_GRAPH block, timed on the second stream); enabled (and reset) by
  * pm_set_timing.  enable = 0 turns timing off; enable = s >= 1 times the
  * kernels of every s-th step of pm_run (steps whose index is a multiple of
  * s), so a long run can be timed with few events. */

@@ -52,6 +52,12 @@ class Simulation:
         self.monitors: list = []
         self.step_count = 0
         self.current_time = 0.0
+        # replay blocks of steps as captured HIP graphs (True) or launch
+        # every step's kernels directly (False, the default: on the 540 x 290
+        # strip both take the same time, 12.5 us per textbook step, because
+        # the device time of its four dependent kernels, not the host's
+        # launches, sets the pace); the results are the same bits either way
+        self.graph: Optional[bool] = False
 
     # -- set-up -------------------------------------------------------------------
     def add_source(self, source) -> None:
@@ -163,8 +169,9 @@ class Simulation:
         ncols = sum(int(np.prod(sh)) for m in dev_mons for _, _, sh in m.device_spec()["boxes"])
         any_td = any(m.device_spec()["time_domain"] for m in dev_mons)
         # rows per spectral block: FP32 sums each block on the matrix cores
-        # before the compensated add, so its blocks stay short
-        T = 128 if esz == 4 else 512
+        # before the compensated add, so its blocks stay short; a small grid
+        # replays blocks as graphs, whose length is the block's
+        T = 128 if esz == 4 or self._small_grid() else 512
         if any_td and ncols:
             T = int(max(1, min(T, steps, (256 << 20) // max(1, ncols * esz))))
         return eng, times_arr, terms, py_sources, dev_mons, py_mons, any_td, T
@@ -203,6 +210,8 @@ class Simulation:
                 m._note_dt(dt)
         flags = N.RUN_SOURCES | N.RUN_SAMPLE | N.RUN_DFT
         per_step = bool(py_sources or py_mons)
+        if self._use_graph(eng, T):
+            flags |= N.RUN_GRAPH
         n = step0
         end = step0 + steps
         while n < end:
@@ -252,6 +261,17 @@ class Simulation:
         for b in eng.mon:
             if b.acc is not None:
                 b.monitor._ingest_spectra(eng.spectra(b))
+
+    def _use_graph(self, eng, T: int) -> bool:
+        if not self.graph or T % 2:
+            return False
+        # two steps per launch (large grids) are not captured
+        return eng.ctx.fused_steps() != 2
+
+    def _small_grid(self) -> bool:
+        """Launch-bound: 2-D, or 3-D up to 2^21 cells."""
+        g = self.grid.dimensions
+        return self.grid.is_2d or int(g[0]) * int(g[1]) * int(g[2]) <= (1 << 21)
 
     # -- results --------------------------------------------------------------------
     def get_field_data(self, monitor, component):

@@ -45,7 +45,11 @@ struct pm_ctx {
   bool span_on = false;   // timing, and the current step is one of those
   std::vector<hipEvent_t> pool;
   size_t pool_used = 0;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> spans[6];
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> spans[7];
+  // graph replay (PM_RUN_GRAPH): one captured block of steps per starting
+  // buffer set, with the run flags it was captured for
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  uint32_t graph_key[2] = {0, 0};
 };
 
 // Scratch of pm_configure_static: one 4 KiB copy on every device the
@@ -70,6 +74,14 @@ int fail(pm_ctx* c, int code, const char* fmt, ...) {
 int hip_check(pm_ctx* c, hipError_t e, const char* what) {
   if (e == hipSuccess) return 0;
   return fail(c, (int)e, "%s: %s", what, hipGetErrorString(e));
+}
+
+void drop_graphs(pm_ctx* c) {
+  for (auto& g : c->graph)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
 }
 
 // Timing events skip the system-scope fence on record: they only time the
@@ -542,6 +554,7 @@ int pm_create(int device, void* stream, pm_ctx** out) {
 int pm_destroy(pm_ctx* c) {
   if (!c) return 0;
   if (c->s2) (void)hipStreamSynchronize(c->s2);
+  drop_graphs(c);
   if (c->fork) (void)hipEventDestroy(c->fork);
   if (c->join) (void)hipEventDestroy(c->join);
   if (c->s2) (void)hipStreamDestroy(c->s2);
@@ -555,6 +568,9 @@ const char* pm_last_error(pm_ctx* c) { return c ? c->err.c_str() : "null context
 int pm_configure(pm_ctx* c, const pm_plan* plan) {
   if (!c || !plan) return PM_EINVAL;
   c->configured = false;
+  // captured graphs hold the old plan's pointers; a replay may still run
+  if (c->s2) (void)hipStreamSynchronize(c->s2);
+  drop_graphs(c);
   const int v = validate(c, *plan);
   if (v) return v;
   const pm_plan& p = *plan;
@@ -681,6 +697,15 @@ int pm_configure(pm_ctx* c, const pm_plan* plan) {
         x.src_merge = false;
     }
   }
+  x.io_merge = x.src_merge && x.n_src > 0 && x.n_sbox > 0 && x.src_max > 0 && x.sbox_max > 0;
+  for (int s = 0; s < x.n_sbox && x.io_merge; ++s)
+    for (int t = 0; t < x.n_src; ++t) {
+      const KSbox& b = x.sbox[s];
+      const KSrc& a = x.src[t];
+      if (a.comp == b.comp && a.i0 < b.i1 && b.i0 < a.i1 && a.j0 < b.j1 && b.j0 < a.j1 &&
+          a.k0 < b.k1 && b.k0 < a.k1)
+        x.io_merge = false;
+    }
   c->sbox_host.assign(x.sbox, x.sbox + x.n_sbox);
   c->src_host.assign(x.src, x.src + x.n_src);
   if (x.n_sbox) {
@@ -765,19 +790,32 @@ namespace {
 
 // Sample step n from the current state (after its sources) and sum a
 // completed block; a gap in the sampled steps sums (or drops) the block.
-int sample_step(pm_ctx* c, int64_t n, bool dft) {
-  const pm_plan& p = c->plan;
-  int r;
+// Before step n is sampled: samples pending from a block that does not
+// continue at n are summed (or dropped without spectra).
+int sample_pre(pm_ctx* c, int64_t n, bool dft) {
   if (c->pend_n && n != c->pend0 + c->pend_n) {
-    if (dft) {
-      if ((r = flush_pending(c))) return r;
-    } else {
-      c->pend_n = 0;
-    }
+    if (dft) return flush_pending(c);
+    c->pend_n = 0;
   }
+  return 0;
+}
+
+// After step n is sampled: it joins the pending block, which is summed
+// when it completes.
+int sample_post(pm_ctx* c, int64_t n, bool dft);
+
+int sample_step(pm_ctx* c, int64_t n, bool dft) {
+  int r;
+  if ((r = sample_pre(c, n, dft))) return r;
   const KAux& ax = c->parity ? c->aux2 : c->aux;
   launch_sample(ax, n, c->stream);
   if ((r = launch_status(c, "sample launch"))) return r;
+  return sample_post(c, n, dft);
+}
+
+int sample_post(pm_ctx* c, int64_t n, bool dft) {
+  const pm_plan& p = c->plan;
+  int r;
   if (c->pend_n == 0) c->pend0 = n;
   ++c->pend_n;
   if ((n + 1) % p.sample_T == 0) {
@@ -787,6 +825,181 @@ int sample_step(pm_ctx* c, int64_t n, bool dft) {
       c->pend_n = 0;
     }
   }
+  return 0;
+}
+
+// ---------------------------------------------------------- graph replay
+// Steps per replayed block when nothing is sampled (with sampling a block is
+// sample_T steps, so its samples fill rows 0 .. sample_T-1 of X and its
+// spectra are summed at its end).  Even: a block of ping-pong steps ends in
+// the buffer set it started from.
+constexpr int32_t kGraphSteps = 32;
+
+int64_t* step_clock(const pm_ctx* c) {
+  return (int64_t*)((char*)c->plan.scratch + kScratchClock);
+}
+
+// One step launched directly: fields (in place, or one ping-pong step),
+// sources, samples.
+int eager_step(pm_ctx* c, int64_t n, bool in_place, bool fused, bool src, bool smp, bool dft) {
+  Span whole(c, 2);
+  int r;
+  if (fused && !in_place) {
+    r = do_fused(c);
+  } else {
+    r = do_h(c);
+    if (!r) r = do_e(c);
+  }
+  if (r) return r;
+  const KAux& ax = c->parity ? c->aux2 : c->aux;
+  if (src && smp && ax.io_merge) {
+    if ((r = sample_pre(c, n, dft))) return r;
+    launch_sources_sample(ax, n, n, c->stream);
+    if ((r = launch_status(c, "source and sample launch"))) return r;
+    return sample_post(c, n, dft);
+  }
+  if (src) {
+    launch_sources(ax, n, c->stream);
+    if ((r = launch_status(c, "source launch"))) return r;
+  }
+  if (smp && (r = sample_step(c, n, dft))) return r;
+  return 0;
+}
+
+// Capture one block of G steps that starts in buffer set `par`, on the
+// second stream: per step the fields, the sources (table row from the step
+// clock, which holds the block's first step) and the samples (row k of X);
+// then the block's spectra (phasor columns from the clock) and the clock
+// advanced by G.  Kernels only: nothing in the graph clears memory.
+int capture_block(pm_ctx* c, int par, int32_t G, bool fused, bool src, bool smp, bool dft) {
+  const pm_plan& p = c->plan;
+  int64_t* clk = step_clock(c);
+  hipStream_t s0 = c->stream;
+  const int par0 = c->parity;
+  const bool span0 = c->span_on;
+  c->stream = c->s2;
+  c->parity = par;
+  c->span_on = false;
+  hipGraph_t g = nullptr;
+  int r = hip_check(c, hipStreamBeginCapture(c->s2, hipStreamCaptureModeThreadLocal),
+                    "begin capture");
+  if (!r) {
+    for (int32_t k = 0; k < G && !r; ++k) {
+      if (fused) {
+        r = do_fused(c);
+      } else {
+        r = do_h(c);
+        if (!r) r = do_e(c);
+      }
+      if (r) break;
+      const KAux& ax = c->parity ? c->aux2 : c->aux;
+      if (src && smp && ax.io_merge) {
+        launch_sources_sample(ax, k, k, c->stream, clk);
+        r = launch_status(c, "source and sample launch");
+        continue;
+      }
+      if (src) {
+        launch_sources(ax, k, c->stream, clk);
+        r = launch_status(c, "source launch");
+      }
+      if (!r && smp) {
+        launch_sample(ax, k, c->stream);
+        r = launch_status(c, "sample launch");
+      }
+    }
+    if (!r && dft) {
+      for (const KDft& d : c->dft)
+        launch_dft(d, p.X, p.ncols, p.W_re, p.W_im, p.table_len, p.table_step0, G,
+                   p.precision, c->stream, clk);
+      r = launch_status(c, "DFT launch");
+    }
+    if (!r) {
+      launch_clock(clk, G, true, c->stream);
+      r = launch_status(c, "clock launch");
+    }
+    const hipError_t e = hipStreamEndCapture(c->s2, &g);
+    if (!r) r = hip_check(c, e, "end capture");
+  }
+  c->stream = s0;
+  c->parity = par0;
+  c->span_on = span0;
+  if (!r)
+    r = hip_check(c, hipGraphInstantiate(&c->graph[par], g, nullptr, nullptr, 0),
+                  "graph instantiate");
+  if (g) (void)hipGraphDestroy(g);
+  return r;
+}
+
+// nrep blocks of G steps from step n0 as graph replays on the second
+// stream, ordered after the caller's stream's work and before its next.
+int replay_blocks(pm_ctx* c, int64_t n0, int32_t nrep, int32_t G, bool fused, bool src,
+                  bool smp, bool dft) {
+  const int par = c->parity;
+  const uint32_t key = 1u | (fused ? 2u : 0u) | (src ? 4u : 0u) | (smp ? 8u : 0u) |
+                       (dft ? 16u : 0u) | ((uint32_t)G << 5);
+  int r;
+  if (!c->graph[par] || c->graph_key[par] != key) {
+    if (c->graph[par]) {
+      (void)hipGraphExecDestroy(c->graph[par]);
+      c->graph[par] = nullptr;
+    }
+    if ((r = capture_block(c, par, G, fused, src, smp, dft))) return r;
+    c->graph_key[par] = key;
+  }
+  launch_clock(step_clock(c), n0, false, c->stream);
+  if ((r = launch_status(c, "clock launch"))) return r;
+  if ((r = hip_check(c, hipEventRecord(c->fork, c->stream), "fork record"))) return r;
+  if ((r = hip_check(c, hipStreamWaitEvent(c->s2, c->fork, 0), "fork wait"))) return r;
+  for (int32_t i = 0; i < nrep; ++i) {
+    hipEvent_t a = c->timing ? take_event(c) : nullptr;
+    if (a) (void)hipEventRecord(a, c->s2);
+    if ((r = hip_check(c, hipGraphLaunch(c->graph[par], c->s2), "graph launch"))) return r;
+    if (a) {
+      hipEvent_t b = take_event(c);
+      if (b) {
+        (void)hipEventRecord(b, c->s2);
+        c->spans[6].push_back({a, b});
+      }
+    }
+  }
+  if ((r = hip_check(c, hipEventRecord(c->join, c->s2), "join record"))) return r;
+  return hip_check(c, hipStreamWaitEvent(c->stream, c->join, 0), "join wait");
+}
+
+// pm_run with PM_RUN_GRAPH: the odd first step of an odd fused run in place
+// (as the direct path does), direct steps up to a block start, whole blocks
+// replayed, the rest direct.
+int run_graph(pm_ctx* c, int64_t step0, int32_t nsteps, int32_t G, bool fused, bool src,
+              bool smp, bool dft) {
+  int32_t s = 0;
+  int r;
+  auto direct = [&](int32_t k) {
+    c->span_on = c->timing && (step0 + k) % c->timing_stride == 0;
+    return eager_step(c, step0 + k, fused && k == 0 && (nsteps & 1), fused, src, smp, dft);
+  };
+  while (s < nsteps && ((fused && s == 0 && (nsteps & 1)) || (smp && (step0 + s) % G != 0))) {
+    if ((r = direct(s))) return r;
+    ++s;
+  }
+  const int32_t nrep = (nsteps - s) / G;
+  if (nrep > 0) {
+    // samples of a block that does not continue here are summed first (as
+    // sample_step does); a block start has none of its own pending
+    if (smp && c->pend_n) {
+      if (dft) {
+        if ((r = flush_pending(c))) return r;
+      } else {
+        c->pend_n = 0;
+      }
+    }
+    if ((r = replay_blocks(c, step0 + s, nrep, G, fused, src, smp, dft))) return r;
+    s += nrep * G;
+  }
+  while (s < nsteps) {
+    if ((r = direct(s))) return r;
+    ++s;
+  }
+  c->span_on = c->timing;
   return 0;
 }
 
@@ -801,7 +1014,6 @@ int pm_run(pm_ctx* c, int64_t step0, int32_t nsteps, uint32_t flags) {
   const bool src = (flags & PM_RUN_SOURCES) && p.n_src > 0 && !io_only;
   const bool smp = (flags & PM_RUN_SAMPLE) && p.n_sbox > 0;
   const bool dft = (flags & PM_RUN_DFT) && smp && !c->dft.empty();
-  if (flags & PM_RUN_GRAPH) return fail(c, PM_ENOTSUP, "graph replay is not built");
   if (io_only && nsteps > 1)
     return fail(c, PM_EINVAL, "PM_RUN_IO_ONLY samples one step (nsteps %d)", nsteps);
   if (src || dft) {
@@ -817,6 +1029,12 @@ int pm_run(pm_ctx* c, int64_t step0, int32_t nsteps, uint32_t flags) {
   }
   int fvec = 0, fw = 0, fty = 0;
   const bool fused = c->fused_ok && c->fuse && fused_shape(c, &fvec, &fw, &fty);
+  if (flags & PM_RUN_GRAPH) {
+    const int32_t G = smp ? p.sample_T : kGraphSteps;
+    if (G % 2) return fail(c, PM_EINVAL, "graph replay needs an even sample_T (%d)", G);
+    if (!c->s2) return fail(c, PM_ENOTSUP, "graph replay needs the context's second stream");
+    return run_graph(c, step0, nsteps, G, fused, src, smp, dft);
+  }
   // two steps per launch for the steps that fill pairs of passes (an even
   // number of passes, so the E-side CPML memory ends in its first set); the
   // one to three steps left over go first: an odd one unfused in place, a
@@ -842,26 +1060,11 @@ int pm_run(pm_ctx* c, int64_t step0, int32_t nsteps, uint32_t flags) {
       s += 2;
       continue;
     }
-    {
-      Span whole(c, 2);
-      // fused runs keep an even number of ping-pong steps: an odd run does
-      // its first step in place, so the call ends with the state in the
-      // first set
-      const int32_t one = two ? head : nsteps;
-      if (fused && !(s == 0 && (one & 1))) {
-        r = do_fused(c);
-      } else {
-        r = do_h(c);
-        if (!r) r = do_e(c);
-      }
-      if (r) return r;
-      const KAux& ax = c->parity ? c->aux2 : c->aux;
-      if (src) {
-        launch_sources(ax, n, c->stream);
-        if ((r = launch_status(c, "source launch"))) return r;
-      }
-      if (smp && (r = sample_step(c, n, dft))) return r;
-    }
+    // fused runs keep an even number of ping-pong steps: an odd run does
+    // its first step in place, so the call ends with the state in the first
+    // set
+    const int32_t one = two ? head : nsteps;
+    if ((r = eager_step(c, n, s == 0 && (one & 1), fused, src, smp, dft))) return r;
     ++s;
   }
   c->span_on = c->timing;
@@ -907,6 +1110,9 @@ int pm_set_tuning(pm_ctx* c, int fuse, int ty, int kchunk, int vec, int order) {
   if (fuse < 0 || fuse > 2) return fail(c, PM_EINVAL, "fuse must be 0, 1 or 2");
   if (fuse == 2 && (ty != 4 || vec != 0))
     return fail(c, PM_EINVAL, "two steps per launch take ty 4 and vec 0 (auto)");
+  // captured graphs hold the old launch shape
+  if (c->s2) (void)hipStreamSynchronize(c->s2);
+  drop_graphs(c);
   c->fuse = fuse;
   c->tune_ty = ty;
   c->tune_kchunk = kchunk;
@@ -940,8 +1146,9 @@ int pm_set_timing(pm_ctx* c, int enable) {
 }
 
 int pm_get_timing(pm_ctx* c, int which, double* avg_ms, int64_t* count) {
-  if (!c || which < 0 || which > 5 || !avg_ms || !count) return PM_EINVAL;
+  if (!c || which < 0 || which > 6 || !avg_ms || !count) return PM_EINVAL;
   int r = hip_check(c, hipStreamSynchronize(c->stream), "stream sync");
+  if (!r && c->s2) r = hip_check(c, hipStreamSynchronize(c->s2), "stream sync");
   if (r) return r;
   double tot = 0;
   int64_t n = 0;

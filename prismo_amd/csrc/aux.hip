@@ -17,16 +17,24 @@
 // otherwise take eight launches per step.  Sources whose boxes of one
 // component overlap keep one launch per box, so their adds happen in the
 // reference's order.
+#include <algorithm>
+
 #include "pm_internal.h"
 
 namespace pm {
 
+// Table row of a source: `row`, plus the step clock when one is given (a
+// captured step replays with the clock holding its first step).
+__device__ __forceinline__ int64_t source_row(int64_t row, const int64_t* clock) {
+  return clock ? row + *clock : row;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_source(const KSrc s, int64_t row, int64_t sy,
-                                                int64_t sz, T* F) {
+                                                int64_t sz, T* F, const int64_t* clock) {
   const int ni = s.i1 - s.i0, nj = s.j1 - s.j0, nk = s.k1 - s.k0;
   const int64_t n = (int64_t)ni * nj * nk;
-  const double a = s.table[row];
+  const double a = s.table[source_row(row, clock)];
   for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
        c += (int64_t)gridDim.x * blockDim.x) {
     const int i = (int)(c % ni);
@@ -67,12 +75,12 @@ __device__ __forceinline__ T* field_of(int c, T* f0, T* f1, T* f2, T* f3, T* f4,
 template <typename T>
 __global__ __launch_bounds__(256) void k_source_all(const KSrc* tab, int64_t row, int64_t sy,
                                                     int64_t sz, T* f0, T* f1, T* f2, T* f3,
-                                                    T* f4, T* f5) {
+                                                    T* f4, T* f5, const int64_t* clock) {
   const KSrc s = tab[blockIdx.y];  // a kernel argument: global memory
   const int ni = s.i1 - s.i0, nj = s.j1 - s.j0, nk = s.k1 - s.k0;
   const int64_t n = (int64_t)ni * nj * nk;
   T* F = field_of<T>(s.comp, f0, f1, f2, f3, f4, f5);
-  const double a = ((gptr<const double>)s.table)[row];
+  const double a = ((gptr<const double>)s.table)[source_row(row, clock)];
   const gptr<const double> prof = (gptr<const double>)s.profile;
   for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
        c += (int64_t)gridDim.x * blockDim.x) {
@@ -108,6 +116,53 @@ __global__ __launch_bounds__(256) void k_sample_all(const KSbox* tab, int64_t sy
   }
 }
 
+// Sources (blockIdx.y < n_src, as k_source_all) and samples (the rest, as
+// k_sample_all) of one step in one launch; only for tables where no sample
+// box meets a source box of its component.
+template <typename T>
+__global__ __launch_bounds__(256) void k_source_sample(const KSrc* stab, int n_src,
+                                                       const KSbox* btab, int64_t row,
+                                                       int64_t sy, int64_t sz, T* f0, T* f1,
+                                                       T* f2, T* f3, T* f4, T* f5, T* Xrow,
+                                                       const int64_t* clock) {
+  const int y = blockIdx.y;
+  if (y < n_src) {
+    const KSrc s = stab[y];
+    const int ni = s.i1 - s.i0, nj = s.j1 - s.j0, nk = s.k1 - s.k0;
+    const int64_t n = (int64_t)ni * nj * nk;
+    T* F = field_of<T>(s.comp, f0, f1, f2, f3, f4, f5);
+    const double a = ((gptr<const double>)s.table)[source_row(row, clock)];
+    const gptr<const double> prof = (gptr<const double>)s.profile;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+         c += (int64_t)gridDim.x * blockDim.x) {
+      const int i = (int)(c % ni);
+      const int64_t r = c / ni;
+      const int j = (int)(r % nj), k = (int)(r / nj);
+      double v = a;
+      if (s.kind == 1) {
+        v = a * prof[c];
+        if (s.divisor != 0.0) v = v / s.divisor;
+      }
+      const int64_t off =
+          (int64_t)(s.i0 + i) + (int64_t)(s.j0 + j) * sy + (int64_t)(s.k0 + k) * sz;
+      F[off] = F[off] + (T)v;
+    }
+    return;
+  }
+  const KSbox b = btab[y - n_src];
+  const int ni = b.i1 - b.i0, nj = b.j1 - b.j0, nk = b.k1 - b.k0;
+  const int64_t n = (int64_t)ni * nj * nk;
+  const T* F = field_of<T>(b.comp, f0, f1, f2, f3, f4, f5);
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(c % ni);
+    const int64_t r = c / ni;
+    const int j = (int)(r % nj), k = (int)(r / nj);
+    const int64_t off = (int64_t)(b.i0 + i) + (int64_t)(b.j0 + j) * sy + (int64_t)(b.k0 + k) * sz;
+    Xrow[b.col0 + c] = F[off];
+  }
+}
+
 static int blocks_for(int64_t n) {
   int64_t b = (n + 255) / 256;
   if (b < 1) b = 1;
@@ -116,10 +171,10 @@ static int blocks_for(int64_t n) {
 }
 
 template <typename T>
-static void launch_source_all(const KAux& a, int64_t row, hipStream_t s) {
+static void launch_source_all(const KAux& a, int64_t row, hipStream_t s, const int64_t* clock) {
   T* const* F = (T* const*)a.F;
   hipLaunchKernelGGL(k_source_all<T>, dim3(blocks_for(a.src_max), a.n_src), dim3(256), 0, s,
-                     a.src_dev, row, a.sy, a.sz, F[0], F[1], F[2], F[3], F[4], F[5]);
+                     a.src_dev, row, a.sy, a.sz, F[0], F[1], F[2], F[3], F[4], F[5], clock);
 }
 
 template <typename T>
@@ -131,13 +186,13 @@ static void launch_sample_all(const KAux& a, int64_t row, hipStream_t s) {
                      (T*)a.X + row * (int64_t)a.ncols);
 }
 
-void launch_sources(const KAux& a, int64_t step, hipStream_t s) {
+void launch_sources(const KAux& a, int64_t step, hipStream_t s, const int64_t* clock) {
   const int64_t row = step - a.table_step0;
   if (a.n_src > 1 && a.src_merge && a.src_max > 0) {
     if (a.precision == PM_FP64)
-      launch_source_all<double>(a, row, s);
+      launch_source_all<double>(a, row, s, clock);
     else
-      launch_source_all<float>(a, row, s);
+      launch_source_all<float>(a, row, s, clock);
     return;
   }
   for (int t = 0; t < a.n_src; ++t) {
@@ -146,10 +201,28 @@ void launch_sources(const KAux& a, int64_t step, hipStream_t s) {
     if (n <= 0) continue;
     if (a.precision == PM_FP64)
       hipLaunchKernelGGL(k_source<double>, dim3(blocks_for(n)), dim3(256), 0, s, src, row,
-                         a.sy, a.sz, (double*)a.F[src.comp]);
+                         a.sy, a.sz, (double*)a.F[src.comp], clock);
     else
       hipLaunchKernelGGL(k_source<float>, dim3(blocks_for(n)), dim3(256), 0, s, src, row,
-                         a.sy, a.sz, (float*)a.F[src.comp]);
+                         a.sy, a.sz, (float*)a.F[src.comp], clock);
+  }
+}
+
+void launch_sources_sample(const KAux& a, int64_t step, int64_t row, hipStream_t s,
+                           const int64_t* clock) {
+  const int64_t srow = step - a.table_step0;
+  const int64_t xrow = row % a.sample_T;
+  const dim3 grid(blocks_for(std::max(a.src_max, a.sbox_max)), a.n_src + a.n_sbox);
+  if (a.precision == PM_FP64) {
+    double* const* F = (double* const*)a.F;
+    hipLaunchKernelGGL(k_source_sample<double>, grid, dim3(256), 0, s, a.src_dev, a.n_src,
+                       a.sbox_dev, srow, a.sy, a.sz, F[0], F[1], F[2], F[3], F[4], F[5],
+                       (double*)a.X + xrow * (int64_t)a.ncols, clock);
+  } else {
+    float* const* F = (float* const*)a.F;
+    hipLaunchKernelGGL(k_source_sample<float>, grid, dim3(256), 0, s, a.src_dev, a.n_src,
+                       a.sbox_dev, srow, a.sy, a.sz, F[0], F[1], F[2], F[3], F[4], F[5],
+                       (float*)a.X + xrow * (int64_t)a.ncols, clock);
   }
 }
 
@@ -175,6 +248,14 @@ void launch_sample(const KAux& a, int64_t step, hipStream_t s) {
                          a.sz, (const float*)a.F[b.comp],
                          (float*)a.X + row * (int64_t)a.ncols);
   }
+}
+
+__global__ void k_step_clock(int64_t* clock, int64_t v, int add) {
+  if (threadIdx.x == 0) *clock = add ? *clock + v : v;
+}
+
+void launch_clock(int64_t* clock, int64_t v, bool add, hipStream_t s) {
+  hipLaunchKernelGGL(k_step_clock, dim3(1), dim3(64), 0, s, clock, v, add ? 1 : 0);
 }
 
 }  // namespace pm

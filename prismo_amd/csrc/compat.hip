@@ -25,39 +25,52 @@
 // np.max of an array holding NaN is NaN and NaN > 0 is false, so a gate is
 // "no NaN and some non-zero".  The TE E update is skipped only when Ex, Ey
 // and Hz are all exactly zero, where it would write +0 back (same bits).
+#include <algorithm>
+
 #include "pm_internal.h"
 
 namespace pm {
 
 // flags[2a] = some element non-zero and not NaN, flags[2a+1] = some NaN,
-// for a = 0: Ez, 1: Ex, 2: Ey, 3: Hz (2-D plane, padded layout).
+// for a = 0: Ez, 1: Ex, 2: Ey, 3: Hz (2-D plane, padded layout), cleared
+// and recomputed before every H update.
 __global__ void k_flags_clear(int* f) {
   if (threadIdx.x < 8) f[threadIdx.x] = 0;
 }
 
+constexpr int kFlagRows = 4;
+
+// One column i per thread and kFlagRows rows per workgroup (the loads of a
+// thread have no loop-carried address, so they overlap); a wave ORs a flag
+// only when its cells set it and it was not yet seen set (every wave
+// ORing into the same eight words serialises the kernel; a stale read only
+// costs an atomic that changes nothing).
 __global__ __launch_bounds__(256) void k_flags2d(const KGrid g) {
   const double* A[4] = {(const double*)g.E[2], (const double*)g.E[0],
                         (const double*)g.E[1], (const double*)g.H[2]};
   const int ex[4] = {g.nx - 1, g.nx, g.nx - 1, g.nx};
   const int ey[4] = {g.ny - 1, g.ny - 1, g.ny, g.ny};
-  bool nz[4] = {false, false, false, false}, nn[4] = {false, false, false, false};
-  const int64_t n = (int64_t)g.nx * g.ny;
-  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
-       c += (int64_t)gridDim.x * blockDim.x) {
-    const int i = (int)(c % g.nx), j = (int)(c / g.nx);
+  unsigned bits = 0;  // bit 2a: non-zero and not NaN, bit 2a+1: NaN
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j0 = blockIdx.y * kFlagRows;
+#pragma unroll
+  for (int r = 0; r < kFlagRows; ++r) {
+    const int j = j0 + r;
     const int64_t off = (int64_t)i + (int64_t)j * g.sy;
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
       if (A[a] == nullptr || i >= ex[a] || j >= ey[a]) continue;
       const double v = A[a][off];
-      if (v != v) nn[a] = true;
-      else if (v != 0.0) nz[a] = true;
+      if (v != v) bits |= 2u << (2 * a);
+      else if (v != 0.0) bits |= 1u << (2 * a);
     }
   }
+  for (int m = 32; m >= 1; m >>= 1) bits |= (unsigned)__shfl_xor((int)bits, m);
+  if ((threadIdx.x % warpSize) == 0 && bits) {
+    volatile const int* fv = g.flags;
 #pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    if (__any(nz[a]) && (threadIdx.x % warpSize) == 0) atomicOr(&g.flags[2 * a], 1);
-    if (__any(nn[a]) && (threadIdx.x % warpSize) == 0) atomicOr(&g.flags[2 * a + 1], 1);
+    for (int b = 0; b < 8; ++b)
+      if (((bits >> b) & 1u) && !fv[b]) atomicOr(&g.flags[b], 1);
   }
 }
 
@@ -209,10 +222,8 @@ __global__ __launch_bounds__(kBX* kBY) void k_compat_e3d(const KGrid g) {
 
 void launch_compat_flags2d(const KGrid& g, hipStream_t s) {
   hipLaunchKernelGGL(k_flags_clear, dim3(1), dim3(64), 0, s, g.flags);
-  const int64_t n = (int64_t)g.nx * g.ny;
-  int blocks = (int)((n + 255) / 256);
-  if (blocks > 1024) blocks = 1024;
-  hipLaunchKernelGGL(k_flags2d, dim3(blocks), dim3(256), 0, s, g);
+  const dim3 grid((g.nx + 255) / 256, (g.ny + kFlagRows - 1) / kFlagRows);
+  hipLaunchKernelGGL(k_flags2d, grid, dim3(256), 0, s, g);
 }
 
 void launch_compat_h(const KGrid& g, int ck, hipStream_t s) {

@@ -78,6 +78,7 @@ struct KGrid {
 
 // scratch layout (caller-owned, PM_SCRATCH_BYTES)
 constexpr int kScratchFlags = 0;
+constexpr int kScratchClock = 512;  // step clock of graph-replayed runs (int64)
 constexpr int kScratchAde = 1024;
 constexpr int kScratchSbox = 2048;  // sample boxes (merged sampling launch)
 constexpr int kScratchSrc = 2816;   // source boxes (merged source launch)
@@ -153,6 +154,10 @@ struct KAux {
   const KSrc* src_dev;
   int64_t sbox_max, src_max;  // cells of the largest box
   bool src_merge;
+  // sources and samples in one launch: merged sources, and no sample box
+  // meets a source box of its component (a sample then never reads a cell
+  // the same launch adds to)
+  bool io_merge;
 };
 
 // One step's buffer pair for the fused kernel.
@@ -192,8 +197,17 @@ bool fused2_grid_ok(const KGrid& g);  // x CPML layers fit the kernel's LDS slot
 // The plan's arrays are laid out as the fused kernel addresses them
 // (packed field sets, Cb, CPML memory pairs and profiles within 4 GiB).
 bool fused_addressable(const KGrid& g, const FusedBufs& fb, int esz);
-void launch_sources(const KAux& a, int64_t step, hipStream_t s);
+// With `clock`, `step` is relative: the table row is step + *clock - table_step0.
+void launch_sources(const KAux& a, int64_t step, hipStream_t s,
+                    const int64_t* clock = nullptr);
+// *clock = v (add false) or *clock += v, in one thread of a kernel (so it can
+// be captured in a graph).
+void launch_clock(int64_t* clock, int64_t v, bool add, hipStream_t s);
 void launch_sample(const KAux& a, int64_t step, hipStream_t s);
+// Sources of step `step` (relative with `clock`, as launch_sources) and the
+// samples into X row `row`, one launch (a.io_merge must hold).
+void launch_sources_sample(const KAux& a, int64_t step, int64_t row, hipStream_t s,
+                           const int64_t* clock = nullptr);
 void launch_dft(const KDft& d, const void* X, int ncols, const void* W_re, const void* W_im,
                 int64_t table_len, int64_t wcol0, int rows, int precision, hipStream_t s,
                 const int64_t* wclock = nullptr);
