@@ -33,8 +33,11 @@ reference-arithmetic and textbook schemes on the GPU, beside the O0 NumPy
 restatement of the reference's step on one host core (a bounded number of
 steps); small grids are launch-bound, so it reports the microseconds per
 step.  ``waveguide_3d``: the dispersive waveguide (480 x 240 x 160 cells,
-Lorentz core, Drude pad, mode source, 64-frequency DFT plane, 6000 steps) in
-FP32 and FP64 with the relative L2 difference of the plane's spectra.
+Lorentz core, Drude pad, mode source, 64-frequency DFT plane, 6000 steps)
+with FP32 fields and FP32 Cb, FP32 fields and FP16 Cb, and FP64: the
+relative L2 difference of each FP32 run's plane spectra from FP64's, and
+each step kernel's roofline (``waveguide_alg_bytes`` over its HIP-event
+time).
 ``voxel_fp64``: the voxel workload with FP64 fields and coefficients (the
 plain method), the same steps: throughput, its roofline (120 B per cell,
 64 B per CPML cell-axis) and the relative L2 difference over all six
@@ -492,10 +495,32 @@ def strip_bench(cpu_steps=60):
     return out
 
 
+def waveguide_alg_bytes(sim, esz: int, csz: int) -> int:
+    """Algorithmic bytes of one waveguide step: six field components read
+    and written and three Cb per cell, two CPML memory values read and
+    written per (cell, CPML axis), and the dispersive state per dispersive
+    E sample (Lorentz P and Q, Drude J) read and written."""
+    import numpy as np
+
+    g = sim.grid
+    n = [int(v) for v in g.dimensions]
+    cells = n[0] * n[1] * n[2]
+    cp = sim.solver.updater.cpml
+    th = cp.thickness if cp is not None else (0,) * 6
+    slab = sum((th[2 * a] + th[2 * a + 1]) * cells // n[a] for a in range(3))
+    ade = 0
+    for r in sim._ade:
+        cnt = int(np.prod([hi - lo for lo, hi in r.box]))
+        ade += cnt * (4 if r.kind == 1 else 2) * esz
+    return (12 * esz + 3 * csz) * cells + 8 * esz * slab + ade
+
+
 def waveguide_bench(steps=None):
     """Dispersive 3-D waveguide (480 x 240 x 160 cells, Lorentz Si core,
-    Drude pad, grid-mode source, 64-frequency DFT plane): FP32 against FP64
-    over the full run; relative L2 of the plane's spectra and of the flux."""
+    Drude pad, grid-mode source, 64-frequency DFT plane): FP32 fields with
+    FP32 and with FP16 Cb, against FP64 over the full run; relative L2 of
+    the plane's spectra and of the flux, and the step kernel's roofline
+    (algorithmic bytes per step over its HIP-event time)."""
     import numpy as np
     import torch
 
@@ -503,42 +528,63 @@ def waveguide_bench(steps=None):
 
     steps = steps or W.WG_STEPS
     res, spec = {}, {}
-    for name, dt in (("fp32", np.float32), ("fp64", np.float64)):
-        sim, mon, mode, _ = W.waveguide_3d(dtype=dt)
+    for name, dt, storage in (("fp32", np.float32, "auto"), ("fp32_fp16cb", np.float32, "fp16"),
+                              ("fp64", np.float64, "auto")):
+        sim, mon, mode, _ = W.waveguide_3d(dtype=dt, coefficient_storage=storage)
         tuned = sim.autotune() if dt == np.float32 else {"fused": False}
+        eng = sim.solver.updater.engine_for(sim.fields)
         sim.run_steps(2)
+        sim.prepare(steps - 2)
         torch.cuda.synchronize()
+        eng.ctx.set_timing(True, stride=10)
         t0 = time.perf_counter()
         sim.run_steps(steps - 2)
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
+        f_ms, nf = eng.ctx.timing(3)
+        h_ms, nh = eng.ctx.timing(0)
+        e_ms, ne = eng.ctx.timing(1)
+        eng.ctx.set_timing(False)
         cells = int(np.prod(W.WG_DIMS))
         flux = W.waveguide_flux(mon)
         spec[name] = {c: np.stack([np.squeeze(mon.get_frequency_data(c, f))
                                    for f in mon.frequencies]) for c in ("Ey", "Ez", "Hy", "Hz")}
+        esz = 4 if dt == np.float32 else 8
+        alg = waveguide_alg_bytes(sim, esz, 2 if storage == "fp16" else esz)
+        k_ms = f_ms if nf >= nh else h_ms + e_ms
         res[name] = {"gcells_per_s": round(cells * (steps - 2) / el / 1e9, 3),
                      "ms_per_step": round(el / (steps - 2) * 1e3, 4), "flux": flux,
                      "fused": bool(tuned.get("fused", False)),
-                     "grid_neff": float(sim.sources[0].grid_neff)}
+                     "launch": {k: tuned.get(k) for k in ("vec", "ty", "kchunk", "order")},
+                     "grid_neff": float(sim.sources[0].grid_neff),
+                     "roofline": {"bound": "hbm",
+                                  "kernel": ("k_yee_fused (dispersive instance)" if nf >= nh
+                                             else "k_yee_h + k_yee_e"),
+                                  "alg_bytes_per_step": alg, "kernel_ms": round(k_ms, 5),
+                                  "achieved": round(alg / (k_ms * 1e-3) / 1e9, 1),
+                                  "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": round(alg / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
         cols = sum(int(np.prod(a.shape[1:])) for a in spec[name].values())
         print(f"[bench] waveguide {name}: {el:.2f} s for {steps - 2} steps", file=sys.stderr,
               flush=True)
         del sim, mon
         torch.cuda.empty_cache()
-    a, b = spec["fp32"], spec["fp64"]
-    num = sum(np.sum(np.abs(a[c] - b[c]) ** 2, axis=(1, 2)) for c in a)
-    den = sum(np.sum(np.abs(b[c]) ** 2, axis=(1, 2)) for c in a)
-    rel = np.sqrt(num / den)
-    fr = np.abs(res["fp32"]["flux"] / res["fp64"]["flux"] - 1)
-    return {
+    out = {
         "workload": "waveguide_3d: 480 x 240 x 160 cells, Lorentz Si core + Drude pad, "
                     "TE grid-mode source, 64-frequency Ey/Ez/Hy/Hz plane",
         "steps": steps, "plane_columns": cols, "frequencies": 64,
-        "fp32": {k: v for k, v in res["fp32"].items() if k != "flux"},
-        "fp64": {k: v for k, v in res["fp64"].items() if k != "flux"},
-        "spectra_rel_l2_fp32_vs_fp64_max": float(rel.max()),
-        "flux_rel_err_fp32_vs_fp64_max": float(fr.max()),
     }
+    b = spec["fp64"]
+    for name in ("fp32", "fp32_fp16cb"):
+        a = spec[name]
+        num = sum(np.sum(np.abs(a[c] - b[c]) ** 2, axis=(1, 2)) for c in a)
+        den = sum(np.sum(np.abs(b[c]) ** 2, axis=(1, 2)) for c in a)
+        fr = np.abs(res[name]["flux"] / res["fp64"]["flux"] - 1)
+        res[name]["spectra_rel_l2_vs_fp64_max"] = float(np.sqrt(num / den).max())
+        res[name]["flux_rel_err_vs_fp64_max"] = float(fr.max())
+    for name in res:
+        out[name] = {k: v for k, v in res[name].items() if k != "flux"}
+    return out
 
 
 def main():
