@@ -59,9 +59,9 @@ FP64 run with the same (FP16-rounded) Cb and against FP64 with exact Cb.
 those legs (short runs for counter collection).  ``--copy-probe`` runs only a
 256 MiB device copy (the counter calibration
 kernel); ``--batch-probe`` only 8 eager steps of the 256-coupler batch (for
-counter runs on the batch kernel); ``--launch vec,ty,kchunk,order[,steps]``
-fixes the fused launch shape (and one or two steps per launch) instead of
-autotuning.
+counter runs on the batch kernel); ``--launch vec,ty,kchunk,order[,steps[,xs]]``
+fixes the fused launch shape (one or two steps per launch, rows whole or in
+two segments) instead of autotuning.
 """
 
 from __future__ import annotations
@@ -128,7 +128,7 @@ def voxel_fp64(n, steps, warmup, ref):
     out = {"gcells_per_s": round(cells * steps / el / 1e9, 3),
            "ms_per_step": round(el / steps * 1e3, 5),
            "fused": bool(nf),
-           "launch": {k: tuned.get(k) for k in ("vec", "ty", "kchunk", "order")},
+           "launch": {k: tuned.get(k) for k in ("vec", "ty", "kchunk", "order", "xs")},
            "rel_l2_fp32_fp16_vs_fp64_6comp": (num / den) ** 0.5 if den > 0 else None}
     if nf:
         out["roofline"] = {"bound": "hbm", "kernel": "k_yee_fused<double>",
@@ -202,7 +202,8 @@ def measured_traffic(workload_key: str, launch: dict):
             if vals.get("workload") != workload_key:
                 continue
             if (tuple(int(x) for x in vals["launch"].split(",")) == want
-                    and int(vals.get("steps", 1)) == steps):
+                    and int(vals.get("steps", 1)) == steps
+                    and int(vals.get("xs", 1)) == (launch.get("xs") or 1)):
                 return float(vals["bytes"])
         except (KeyError, ValueError, TypeError):
             continue
@@ -531,7 +532,7 @@ def waveguide_bench(steps=None):
     for name, dt, storage in (("fp32", np.float32, "auto"), ("fp32_fp16cb", np.float32, "fp16"),
                               ("fp64", np.float64, "auto")):
         sim, mon, mode, _ = W.waveguide_3d(dtype=dt, coefficient_storage=storage)
-        tuned = sim.autotune() if dt == np.float32 else {"fused": False}
+        tuned = sim.autotune()
         eng = sim.solver.updater.engine_for(sim.fields)
         sim.run_steps(2)
         sim.prepare(steps - 2)
@@ -555,7 +556,7 @@ def waveguide_bench(steps=None):
         res[name] = {"gcells_per_s": round(cells * (steps - 2) / el / 1e9, 3),
                      "ms_per_step": round(el / (steps - 2) * 1e3, 4), "flux": flux,
                      "fused": bool(tuned.get("fused", False)),
-                     "launch": {k: tuned.get(k) for k in ("vec", "ty", "kchunk", "order")},
+                     "launch": {k: tuned.get(k) for k in ("vec", "ty", "kchunk", "order", "xs")},
                      "grid_neff": float(sim.sources[0].grid_neff),
                      "roofline": {"bound": "hbm",
                                   "kernel": ("k_yee_fused (dispersive instance)" if nf >= nh
@@ -658,9 +659,11 @@ def main():
         parts = [int(x) for x in args.launch.split(",")]
         vec, ty, kc, order = parts[:4]
         nstep = parts[4] if len(parts) > 4 else 1
+        xs = parts[5] if len(parts) > 5 else 1
+        eng.ctx.set_row_split(xs)
         eng.ctx.set_tuning(nstep, ty, kc, vec, order)
         tuned = {"steps": nstep, "vec": vec, "ty": ty, "kchunk": kc, "order": order,
-                 "fused": True}
+                 "xs": xs, "fused": True}
     else:
         tuned = sim.autotune()
     print(f"[bench] fused step launch: {tuned}", file=sys.stderr, flush=True)
@@ -698,7 +701,7 @@ def main():
         kern_ms, kname, steps_per_launch = h_ms + e_ms, "k_yee_h + k_yee_e (one step)", 1
     alg_bytes = per_step_bytes * steps_per_launch
     achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
-    launch = {k: tuned.get(k) for k in ("steps", "vec", "ty", "kchunk", "order")}
+    launch = {k: tuned.get(k) for k in ("steps", "vec", "ty", "kchunk", "order", "xs")}
     wkey = f"voxel_3d_{args.n}_fp32_fp16"
     traffic = measured_traffic(wkey, launch)
     finite = all(bool(torch.isfinite(sim.fields[c]).all()) for c in ("Ex", "Ey", "Ez"))

@@ -286,6 +286,17 @@ int pm_dft_flush(pm_ctx* ctx, int64_t step_first, int32_t rows);
  * 8, 0, 2.  Used by the autotuner. */
 int pm_set_tuning(pm_ctx* ctx, int fuse, int ty, int kchunk, int vec, int order);
 
+/* Rows of the fused one-step kernel in workgroups of a whole padded row
+ * (segments = 1) or of half of it (2: the E update of a segment's last cell
+ * recomputes H of the cell to its right; plans with Cb-only coefficients
+ * whose middle cell lies outside the x CPML layers); 0 (default) = whole
+ * rows when a launch shape fits them, else two segments (FP64 512-cell
+ * rows).  The results do not depend on it.  pm_row_split_active: segments
+ * the next fused step takes (0 when it is not fused).  Used by the
+ * autotuner; no counterpart in the reference. */
+int pm_set_row_split(pm_ctx* ctx, int segments);
+int pm_row_split_active(pm_ctx* ctx);
+
 /* Steps per fused launch pm_run will use with the current plan and tuning
  * for a run without monitor sampling: 0 (unfused kernels), 1 or 2. */
 int pm_fused_active(pm_ctx* ctx);

@@ -193,6 +193,8 @@ SIGNATURES = {
     "pm_set_tuning": ([vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
     "pm_fused_active": ([vp], C.c_int),
     "pm_sync": ([vp], C.c_int),
+    "pm_set_row_split": ([vp, C.c_int], C.c_int),
+    "pm_row_split_active": ([vp], C.c_int),
     "pm_set_timing": ([vp, C.c_int], C.c_int),
     "pm_get_timing": ([vp, C.c_int, C.POINTER(f64), C.POINTER(i64)], C.c_int),
 }
@@ -286,6 +288,14 @@ class Context:
 
     def fused_active(self) -> bool:
         return bool(self.lib.pm_fused_active(self.h))
+
+    def set_row_split(self, segments: int = 0):
+        """Fused-kernel rows in 1 or 2 workgroups (0: automatic)."""
+        self._check(self.lib.pm_set_row_split(self.h, int(segments)), "pm_set_row_split")
+
+    def row_split(self) -> int:
+        """Row segments the next fused step takes (0 when not fused)."""
+        return int(self.lib.pm_row_split_active(self.h))
 
     def fused_steps(self) -> int:
         """Steps per fused launch of a run without monitor sampling (0, 1, 2)."""

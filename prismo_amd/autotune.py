@@ -2,7 +2,8 @@
 2-D step (``tune_batch``).
 
 Times each candidate (cells per lane, rows per workgroup, z planes per
-workgroup, tile order, and one or two steps per launch) for a few steps with
+workgroup, tile order, rows whole or in two segments, and one or two steps
+per launch) for a few steps with
 HIP events on the library's own stream, keeps the fastest per step, and
 restores the simulation state it started from:
 fields, CPML memory and dispersive state are snapshotted on the device first,
@@ -91,23 +92,28 @@ def tune(eng, steps: int = 6, warmup: int = 2, candidates: dict | None = None,
     key = _key(eng)
     if key in _CACHE and not force:
         best = _CACHE[key]
+        eng.ctx.set_row_split(best.get("xs", 0))
         eng.ctx.set_tuning(best.get("steps", 1), best["ty"], best["kchunk"], best["vec"],
                            best["order"])
         return dict(best, fused=True, cached=True)
     cand = candidates or CANDIDATES
-    shapes = [(1, vec, ty, kc, order) for vec, ty, kc, order in
-              product(cand["vec"], cand["ty"], cand["kchunk"], cand["order"])]
+    splits = cand.get("xs", (1, 2))
+    shapes = [(1, vec, ty, kc, order, xs) for xs, vec, ty, kc, order in
+              product(splits, cand["vec"], cand["ty"], cand["kchunk"], cand["order"])]
     if candidates is None or "two_step" in candidates:
         c2 = (candidates or {}).get("two_step", CANDIDATES2)
-        shapes += [(2, 0, 4, kc, order) for kc, order in product(c2["kchunk"], c2["order"])]
+        shapes += [(2, 0, 4, kc, order, 1) for kc, order in product(c2["kchunk"], c2["order"])]
     saved = [t.clone() for t in _state_tensors(eng)]
     best = None
     try:
-        for nstep, vec, ty, kc, order in shapes:
+        for nstep, vec, ty, kc, order, xs in shapes:
             try:
+                eng.ctx.set_row_split(xs)
                 eng.ctx.set_tuning(nstep, ty, kc, vec, order)
                 if eng.ctx.fused_steps() != nstep:
                     continue
+                if nstep == 1 and eng.ctx.row_split() != xs:
+                    continue  # the plan cannot take this many segments
                 # whole pairs of passes (4 steps) keep the two-step path on
                 run = lambda n: eng.ctx.run(0, -(-n // 4) * 4, 0)
                 run(warmup)
@@ -120,14 +126,16 @@ def tune(eng, steps: int = 6, warmup: int = 2, candidates: dict | None = None,
                 continue
             if n and (best is None or ms / nstep < best["ms"]):
                 best = {"steps": nstep, "vec": vec, "ty": ty, "kchunk": kc, "order": order,
-                        "ms": ms / nstep, "ms_launch": ms}
+                        "xs": xs, "ms": ms / nstep, "ms_launch": ms}
     finally:
         for t, s in zip(_state_tensors(eng), saved):
             t.copy_(s)
         torch.cuda.synchronize(eng.device)
     if best is None:
+        eng.ctx.set_row_split(0)
         eng.ctx.set_tuning(True, 4, 8, 0, 2)
         return {"fused": False}
+    eng.ctx.set_row_split(best["xs"])
     eng.ctx.set_tuning(best["steps"], best["ty"], best["kchunk"], best["vec"], best["order"])
     _CACHE[key] = best
     _save_file()

@@ -26,6 +26,11 @@ struct pm_ctx {
   // fused ping-pong step: parity 0 = state in plan.E/H/psi_h, 1 = in *2
   bool fused_ok = false;
   int fuse = 1, tune_ty = 4, tune_kchunk = 8, tune_vec = 0, tune_order = 2;
+  // rows in two segments: allowed for this plan (Cb-only coefficients, the
+  // cell right of the first segment outside the x CPML layers), and the
+  // tuning (0 auto, 1 whole rows, 2 two segments)
+  bool split_ok = false;
+  int tune_xs = 0;
   int parity = 0;
   // two steps per launch: plan eligible, and which E-side CPML memory set
   // holds the state (0 = plan.psi_e, 1 = plan.psi_e2)
@@ -336,28 +341,43 @@ int do_e(pm_ctx* c) {
 // Launch shape of the fused kernel: the tuned one, else (rows per workgroup
 // and then cells per lane relaxed) the first that fits the workgroup and LDS
 // limits.  False if none does.
-bool fused_shape(const pm_ctx* c, int* vec, int* w, int* ty) {
-  const int rw = (int)c->plan.sy;  // padded row, 128 / 256 / 512
-  const int prec = c->plan.precision;
-  int prefer = c->tune_vec;
-  if (prefer <= 0) prefer = prec == PM_FP32 ? (rw >= 128 ? 2 : 1) : (rw <= 256 ? 1 : 2);
-  const int vecs[4] = {prefer, 1, 2, 4};
-  const int nv = c->tune_vec > 0 ? 1 : 4;
-  for (int iv = 0; iv < nv; ++iv) {
-    const int v = vecs[iv];
-    if (rw % (64 * v)) continue;
-    for (int t : {c->tune_ty, 2}) {
-      if (fused_supported(prec, v, rw / (64 * v), t)) {
-        *vec = v;
-        *w = rw / (64 * v);
-        *ty = t;
-        return true;
+// Rows in xs segments (pm_set_row_split: 1 or 2; 0 = whole rows when a
+// shape fits them, else two segments).
+bool fused_shape(const pm_ctx* c, int* vec, int* w, int* ty, int* xs) {
+  const int sy = (int)c->plan.sy;  // padded row, 128 / 256 / 512
+  const int prec = c->plan.precision, ck = c->plan.coeff_kind;
+  int opts[2] = {1, 2}, nopt = 2;
+  if (c->tune_xs == 1 || c->tune_xs == 2) {
+    opts[0] = c->tune_xs;
+    nopt = 1;
+  }
+  for (int a = 0; a < nopt; ++a) {
+    const int x = opts[a];
+    if (x == 2 && !c->split_ok) continue;
+    const int rw = sy / x;  // cells per segment
+    int prefer = c->tune_vec;
+    if (prefer <= 0) prefer = prec == PM_FP32 ? (rw >= 128 && x == 1 ? 2 : 1) : (rw <= 256 ? 1 : 2);
+    const int vecs[4] = {prefer, 1, 2, 4};
+    const int nv = c->tune_vec > 0 ? 1 : 4;
+    for (int iv = 0; iv < nv; ++iv) {
+      const int v = vecs[iv];
+      if (rw % (64 * v)) continue;
+      for (int t : {c->tune_ty, 2}) {
+        if (fused_supported(prec, ck, v, rw / (64 * v), t, x)) {
+          *vec = v;
+          *w = rw / (64 * v);
+          *ty = t;
+          *xs = x;
+          return true;
+        }
       }
     }
   }
+  const int prefer = c->tune_vec > 0 ? c->tune_vec : 1;
   *vec = prefer;
-  *w = rw / (64 * prefer);
+  *w = sy / (64 * prefer);
   *ty = c->tune_ty;
+  *xs = 1;
   return false;
 }
 
@@ -392,15 +412,16 @@ int do_fused(pm_ctx* c) {
       fb.psh_out[a][q] = a_in ? p.psi_h2[a][q] : p.psi_h[a][q];
     }
   }
-  int vec, w, ty;
-  fused_shape(c, &vec, &w, &ty);
+  int vec, w, ty, xs;
+  fused_shape(c, &vec, &w, &ty, &xs);
   const int r = p.precision == PM_FP32
                     ? launch_yee_fused_f32(c->g, p.coeff_kind, fb, vec, w, ty,
-                                           c->tune_kchunk, c->tune_order, c->stream)
+                                           c->tune_kchunk, c->tune_order, xs, c->stream)
                     : launch_yee_fused_f64(c->g, p.coeff_kind, fb, vec, w, ty,
-                                           c->tune_kchunk, c->tune_order, c->stream);
+                                           c->tune_kchunk, c->tune_order, xs, c->stream);
   if (r != 0)
-    return fail(c, PM_ENOTSUP, "fused step: no kernel for vec %d, w %d, ty %d", vec, w, ty);
+    return fail(c, PM_ENOTSUP, "fused step: no kernel for vec %d, w %d, ty %d, %d segments",
+                vec, w, ty, xs);
   c->parity ^= 1;
   return launch_status(c, "fused step launch");
 }
@@ -736,6 +757,13 @@ int pm_configure(pm_ctx* c, const pm_plan* plan) {
     int r = hip_check(c, hipStreamSynchronize(c->stream), "table upload sync");
     if (r) return r;
   }
+  {
+    // two row segments: the cell right of the first one (x = sy / 2) must
+    // lie inside the grid and outside the x CPML layers
+    const int gxr = (int)(p.sy / 2);
+    c->split_ok = p.coeff_kind != PM_COEFF_FULL && gxr < p.nx && gxr >= p.pml[0] &&
+                  gxr < p.nx - p.pml[1];
+  }
   c->fused_ok = fused_eligible(p) &&
                 fused_addressable(c->g, fused_bufs(p, true), p.precision == PM_FP64 ? 8 : 4) &&
                 fused_addressable(c->g, fused_bufs(p, false), p.precision == PM_FP64 ? 8 : 4);
@@ -1027,8 +1055,8 @@ int pm_run(pm_ctx* c, int64_t step0, int32_t nsteps, uint32_t flags) {
     if (nsteps == 1 && smp) return sample_step(c, step0, dft);
     return 0;
   }
-  int fvec = 0, fw = 0, fty = 0;
-  const bool fused = c->fused_ok && c->fuse && fused_shape(c, &fvec, &fw, &fty);
+  int fvec = 0, fw = 0, fty = 0, fxs = 1;
+  const bool fused = c->fused_ok && c->fuse && fused_shape(c, &fvec, &fw, &fty, &fxs);
   if (flags & PM_RUN_GRAPH) {
     const int32_t G = smp ? p.sample_T : kGraphSteps;
     if (G % 2) return fail(c, PM_EINVAL, "graph replay needs an even sample_T (%d)", G);
@@ -1100,6 +1128,22 @@ int pm_dft_flush(pm_ctx* c, int64_t step_first, int32_t rows) {
   return flush_rows(c, step_first, rows);
 }
 
+int pm_set_row_split(pm_ctx* c, int segments) {
+  if (!c) return PM_EINVAL;
+  if (segments < 0 || segments > 2) return fail(c, PM_EINVAL, "segments must be 0, 1 or 2");
+  if (c->s2) (void)hipStreamSynchronize(c->s2);
+  drop_graphs(c);
+  c->tune_xs = segments;
+  return 0;
+}
+
+int pm_row_split_active(pm_ctx* c) {
+  if (!c || !c->configured || !c->fused_ok || !c->fuse) return 0;
+  int vec = 0, w = 0, ty = 0, xs = 1;
+  if (!fused_shape(c, &vec, &w, &ty, &xs)) return 0;
+  return xs;
+}
+
 int pm_set_tuning(pm_ctx* c, int fuse, int ty, int kchunk, int vec, int order) {
   if (!c) return PM_EINVAL;
   if (ty < 2 || ty > 4) return fail(c, PM_EINVAL, "ty must be 2, 3 or 4");
@@ -1125,7 +1169,8 @@ int pm_fused_active(pm_ctx* c) {
   if (!c || !c->configured) return 0;
   int vec = 0, w = 0, ty = 0;
   if (!c->fused_ok) return 0;
-  if (!(c->fuse && fused_shape(c, &vec, &w, &ty))) return 0;
+  int xs = 1;
+  if (!(c->fuse && fused_shape(c, &vec, &w, &ty, &xs))) return 0;
   return (c->fuse == 2 && c->fused2_ok) ? 2 : 1;
 }
 

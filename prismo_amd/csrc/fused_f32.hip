@@ -5,14 +5,14 @@
 namespace pm {
 
 int launch_yee_fused_f32(const KGrid& g, int ck, const FusedBufs& fb, int vec, int w, int ty,
-                         int kchunk, int order, hipStream_t s) {
+                         int kchunk, int order, int xs, hipStream_t s) {
   KFz z;
   if (!make_kfz(g, fb, (int)sizeof(float), &z)) return -1;
   if (ck == PM_COEFF_FULL)
-    return launch_fused_ck<float, PM_COEFF_FULL>(g, z, vec, w, ty, kchunk, order, s);
+    return launch_fused_ck<float, PM_COEFF_FULL>(g, z, vec, w, ty, kchunk, order, xs, s);
   if (ck == PM_COEFF_CB)
-    return launch_fused_ck<float, PM_COEFF_CB>(g, z, vec, w, ty, kchunk, order, s);
-  return launch_fused_ck<float, PM_COEFF_CB_F16>(g, z, vec, w, ty, kchunk, order, s);
+    return launch_fused_ck<float, PM_COEFF_CB>(g, z, vec, w, ty, kchunk, order, xs, s);
+  return launch_fused_ck<float, PM_COEFF_CB_F16>(g, z, vec, w, ty, kchunk, order, xs, s);
 }
 
 bool fused2_grid_ok(const KGrid& g) { return g.pml[0] + g.pml[1] <= kF2XSlab; }

@@ -201,20 +201,30 @@ struct TileBox {
   int ty0, ty1, kc0, kc1;
 };
 
-template <typename T, int VEC, int W, int TY>
+template <typename T, int VEC, int W, int TY, int XS>
 struct FusedLds {
   static constexpr int RW = 64 * VEC * W;
   T le[3][TY + 1][RW];     // E(n) of plane k
   T lh[2][3][TY + 1][RW];  // H(n+1/2) of planes with parity 0 / 1
+  T xr[XS > 1 ? TY + 1 : 1];  // Ex(n) of plane k one cell right of the segment
 };
 
-// One tile (TY rows x one z-chunk); b = the workgroup's index within the
-// launch segment of its kind (slab tiles or interior tiles).
-template <typename T, int CK, int VEC, int W, int TY, bool YZ, bool ADE>
+// One tile (TY rows x one z-chunk x one row segment of RW cells); b = the
+// workgroup's index within the launch segment of its kind (slab tiles or
+// interior tiles).
+//
+// Row segments (XS > 1: a padded row of XS * RW cells in XS workgroups).
+// The H update of a segment's first cell takes E(n) of the cell to its left
+// from memory (E(n) is only read this step), and the E update of its last
+// cell takes H(n+1/2) of the cell to its right, which the last lane
+// recomputes from E(n), H(n-1/2) and the CPML memory of that cell with the
+// arithmetic the neighbouring segment uses for it (not stored).  Segment
+// boundaries lie outside the x CPML layers (the host checks).
+template <typename T, int CK, int VEC, int W, int TY, int XS, bool YZ, bool ADE>
 __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kchunk,
                                            int ntile_y, int nchunk, int nblocks, int order,
                                            const TileBox& bx, int b,
-                                           FusedLds<T, VEC, W, TY>& lds, const KAde* tab,
+                                           FusedLds<T, VEC, W, TY, XS>& lds, const KAde* tab,
                                            const T (*tcf)[12]) {
   constexpr int RW = 64 * VEC * W;
   using ABits = std::conditional_t<(12 * VEC > 32), uint64_t, uint32_t>;
@@ -236,6 +246,9 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
     L = (b % 8) * per + b / 8;
   }
   if (L >= nblocks) return;
+  // consecutive tile numbers are the segments of one row tile
+  const int xs = XS > 1 ? L % XS : 0;
+  if constexpr (XS > 1) L /= XS;
   int ty, kc;
   if constexpr (!YZ) {
     const int nty = bx.ty1 - bx.ty0, nkc = bx.kc1 - bx.kc0;
@@ -269,7 +282,12 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
   // the row index is the same for a whole wave: make it scalar so row-wise
   // quantities (y profiles, slab indices) live in SGPRs and load as scalars
   const int w = __builtin_amdgcn_readfirstlane(tid / (64 * W));  // TY = halo row
-  const int x0 = (tid % (64 * W)) * VEC;
+  const int x0 = (tid % (64 * W)) * VEC;  // within the segment (LDS)
+  const int gx0 = xs * RW + x0;             // within the row (memory)
+  // this segment has a segment to its left / right
+  const bool seg_l = XS > 1 && xs > 0;
+  const bool seg_r = XS > 1 && xs + 1 < XS;
+  const bool lane_l = x0 == 0, lane_r = x0 == RW - VEC;
   const int j = ty * TY + w;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const bool row_ok = j < ny;
@@ -290,7 +308,7 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
   T xbh[VEC], xch[VEC], xikh[VEC], xbe[VEC], xce[VEC], xike[VEC];
 #pragma unroll
   for (int e = 0; e < VEC; ++e) {
-    const int i = x0 + e;
+    const int i = gx0 + e;
     sxs[e] = (pml && i < nx) ? fslab(i, nx, g.pml[0], g.pml[1]) : -1;
     lane_pml_x |= sxs[e] >= 0;
     xbh[e] = xbe[e] = (T)1;
@@ -323,11 +341,11 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
   // Planes alternate between two register sets (the loop is unrolled by
   // two), so no in-flight register is ever copied.
   const int jr = min(j, ny - 1);
-  const uint32_t rowoff = (uint32_t)jr * sy + (uint32_t)x0;
-  const uint32_t upoff = (uint32_t)(first_row ? j - 1 : jr) * sy + (uint32_t)x0;
+  const uint32_t rowoff = (uint32_t)jr * sy + (uint32_t)gx0;
+  const uint32_t upoff = (uint32_t)(first_row ? j - 1 : jr) * sy + (uint32_t)gx0;
   // the halo row takes no E update: its Cb loads read the row above, which
   // the tile's last own row fetches at the same time (a cache hit)
-  const uint32_t cboff = (uint32_t)(w < TY ? jr : min(j - 1, ny - 1)) * sy + (uint32_t)x0;
+  const uint32_t cboff = (uint32_t)(w < TY ? jr : min(j - 1, ny - 1)) * sy + (uint32_t)gx0;
   int sxc[VEC];
 #pragma unroll
   for (int e = 0; e < VEC; ++e) sxc[e] = max(sxs[e], 0);
@@ -341,7 +359,7 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
     const int skc = YZ ? max(fslab(kq, nz, g.pml[4], g.pml[5]), 0) : 0;
 #pragma unroll
     for (int e = 0; e < VEC; ++e) {
-      const int ic = min(x0 + e, nx - 1);
+      const int ic = min(gx0 + e, nx - 1);
       const uint32_t ox = fslab_off(g, 0, sxc[e], ic, jr, kq);
       q[0][e] = *zp<T>(z.bc, base[0], ox);
       q[1][e] = *zp<T>(z.bc, base[0] + z.ps[0], ox);
@@ -394,6 +412,25 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
     fld(exm, zp<T>(z.bi, 0u, o));
     fld(eym, zp<T>(z.bi, z.cs, o));
   }
+  // row segments, last lane: Ex(n) of the cell right of the segment on the
+  // previous plane, and that cell's recomputed H(n+1/2) on the previous
+  // plane (which the E update of that plane takes)
+  const int gxr = (xs + 1) * RW;  // the cell right of the segment
+  // the halo cells are the same for every lane of a row: their loads are
+  // scalar loads (the input field set and the H-side CPML memory are not
+  // written by this kernel), so they hold no vector registers
+  const uint32_t rcol = (uint32_t)(seg_r ? min(gxr, nx - 1) : 0);
+  const uint32_t lcol = (uint32_t)(seg_l ? xs * RW - 1 : 0);
+  using CT = const T __attribute__((address_space(4)))*;
+  auto sld = [&](const char* base, uint32_t off, uint32_t elem) __attribute__((always_inline)) {
+    const uint32_t o =
+        (uint32_t)__builtin_amdgcn_readfirstlane((int)(off + elem * (uint32_t)sizeof(T)));
+    return *(CT)(base + o);
+  };
+  T sr_exm = 0, hyh_p = 0, hzh_p = 0;
+  if constexpr (XS > 1) {
+    if (row_ok && kb > 0) sr_exm = sld(z.bi, 0u, (uint32_t)jr * sy + rcol + (uint32_t)(kb - 1) * sz);
+  }
 
   // dispersive boxes this wave's row cuts, per component (uniform), and per
   // lane those whose x range holds cell e: the z-march changes only the
@@ -416,7 +453,7 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
           rc2 |= bit;
 #pragma unroll
         for (int e = 0; e < VEC; ++e)
-          if (x0 + e >= a.i0 && x0 + e < a.i1) xm[e] |= bit;
+          if (gx0 + e >= a.i0 && gx0 + e < a.i1) xm[e] |= bit;
       }
     }
   }
@@ -426,6 +463,29 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
     const bool do_e = k > kb && own;
     const int kk = k - 1;
     const int kq = k > kb ? kk : k;  // where the E-side loads read
+    // (0) row segments: E(n) left of the segment's first cell, and E(n),
+    //     H(n-1/2) and the CPML memory of the cell right of its last one
+    //     (scalar loads, one per row; a segment at the row's end reads
+    //     column 0 or nx-1, unused)
+    T sl_ey = 0, sl_ez = 0;
+    T sr_ex = 0, sr_ey = 0, sr_ez = 0, sr_hy = 0, sr_hz = 0, sr_exu = 0, sr_q2 = 0, sr_q5 = 0;
+    if constexpr (XS > 1) {
+      const uint32_t lo = (uint32_t)jr * sy + lcol + (uint32_t)k * sz;
+      sl_ey = sld(z.bi, z.cs, lo);
+      sl_ez = sld(z.bi, 2u * z.cs, lo);
+      const uint32_t ro = (uint32_t)jr * sy + rcol + (uint32_t)k * sz;
+      sr_ex = sld(z.bi, 0u, ro);
+      sr_ey = sld(z.bi, z.cs, ro);
+      sr_ez = sld(z.bi, 2u * z.cs, ro);
+      sr_hy = sld(z.bi, 4u * z.cs, ro);
+      sr_hz = sld(z.bi, 5u * z.cs, ro);
+      sr_exu = sld(z.bi, 0u, (uint32_t)(first_row ? j - 1 : jr) * sy + rcol + (uint32_t)k * sz);
+      if constexpr (YZ) {
+        const int skh = max(fslab(k, nz, g.pml[4], g.pml[5]), 0);
+        sr_q2 = sld(z.bc, z.phi[1], fslab_off(g, 1, syc, (int)rcol, jr, k));
+        sr_q5 = sld(z.bc, z.phi[2] + z.ps[2], fslab_off(g, 2, skh, (int)rcol, jr, k));
+      }
+    }
     // (1) what this iteration's E update needs, issued ahead of the next
     //     plane's prefetch: vector memory counters retire in order, so the
     //     wait for these does not wait for the prefetch
@@ -487,7 +547,7 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
           for (int e = 0; e < VEC; ++e) {
             const int t = (int)((abits >> (4 * (c * VEC + e))) & 15) - 1;
             const KAde& a = tab[t < 0 ? 0 : t];
-            const uint32_t l = t >= 0 ? (uint32_t)ade_index(a, x0 + e, j, kk) : 0u;
+            const uint32_t l = t >= 0 ? (uint32_t)ade_index(a, gx0 + e, j, kk) : 0u;
             const gptr<const T> pp = t >= 0 ? (gptr<const T>)a.p + l : dummy;
             const gptr<const T> qq = (t >= 0 && a.kind == 1) ? (gptr<const T>)a.q + l : dummy;
             ap[c][e] = *pp;
@@ -504,6 +564,9 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
       le[1][w][x0 + e] = cur.ey[e];
       le[2][w][x0 + e] = cur.ez[e];
     }
+    if constexpr (XS > 1) {
+      if (lane_r) lds.xr[w] = sr_ex;
+    }
     __syncthreads();
     // (4) H(n+1/2) on plane k
     T hx[VEC], hy[VEC], hz[VEC];
@@ -515,12 +578,12 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
       const T db = (T)g.db_const;
 #pragma unroll
       for (int e = 0; e < VEC; ++e) {
-        const int i = x0 + e;
+        const int li = x0 + e, i = gx0 + e;
         const T ex = cur.ex[e], ey = cur.ey[e], ez = cur.ez[e];
-        const T exj = w > 0 ? le[0][w - 1][i] : (first_row ? cur.exu[e] : (T)0);
-        const T ezj = w > 0 ? le[2][w - 1][i] : (first_row ? cur.ezu[e] : (T)0);
-        const T ezi = i > 0 ? le[2][w][i - 1] : (T)0;
-        const T eyi = i > 0 ? le[1][w][i - 1] : (T)0;
+        const T exj = w > 0 ? le[0][w - 1][li] : (first_row ? cur.exu[e] : (T)0);
+        const T ezj = w > 0 ? le[2][w - 1][li] : (first_row ? cur.ezu[e] : (T)0);
+        const T ezi = li > 0 ? le[2][w][li - 1] : (seg_l ? sl_ez : (T)0);
+        const T eyi = li > 0 ? le[1][w][li - 1] : (seg_l ? sl_ey : (T)0);
         T c1 = (ez - ezj) * idy;  // Hx: dEz/dy - dEy/dz
         T c2 = (ey - eym[e]) * idz;
         T d1 = (ex - exm[e]) * idz;  // Hy: dEx/dz - dEz/dx
@@ -577,6 +640,34 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
       fzero(hy);
       fzero(hz);
     }
+    // (4b) row segments: H(n+1/2) of the cell right of the segment, with the
+    //      arithmetic of the H update above (that cell lies outside the x
+    //      CPML layers), kept in registers for the E update of the last cell
+    T hyh = 0, hzh = 0;
+    if constexpr (XS > 1) {
+      if (seg_r && lane_r && row_ok) {
+        const T db = (T)g.db_const;
+        const T exj = w > 0 ? lds.xr[w - 1] : (first_row ? sr_exu : (T)0);
+        T d1 = (sr_ex - sr_exm) * idz;
+        T d2 = (sr_ez - cur.ez[VEC - 1]) * idx;
+        T f1 = (sr_ey - cur.ey[VEC - 1]) * idx;
+        T f2 = (sr_ex - exj) * idy;
+        if (gxr < nx) {
+          if (sys >= 0) {
+            const T p0 = ybh * sr_q2 + ych * f2;
+            f2 = f2 * yikh + p0;
+          }
+          if (szs >= 0) {
+            const T p1 = zh[0] * sr_q5 + zh[1] * d1;
+            d1 = d1 * zh[2] + p1;
+          }
+        }
+        hyh = sr_hy - db * (d1 - d2);
+        hzh = sr_hz - db * (f1 - f2);
+        if (gxr >= nx || j >= ny - 1) hyh = 0;
+        if (gxr >= nx || k >= nz - 1) hzh = 0;
+      }
+    }
     // (5) share H(n+1/2) of plane k
 #pragma unroll
     for (int e = 0; e < VEC; ++e) {
@@ -595,10 +686,14 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
       T ox[VEC], oy[VEC], oz[VEC];
 #pragma unroll
       for (int e = 0; e < VEC; ++e) {
-        const int i = x0 + e;
-        const int ir = i + 1 < RW ? i + 1 : i;
-        const T hyx = lh[p][1][w][ir], hzx = lh[p][2][w][ir];
-        const T hxy = lh[p][0][w + 1][i], hzy = lh[p][2][w + 1][i];
+        const int li = x0 + e, i = gx0 + e;
+        const int ir = li + 1 < RW ? li + 1 : li;
+        T hyx = lh[p][1][w][ir], hzx = lh[p][2][w][ir];
+        if (seg_r && li == RW - 1) {
+          hyx = hyh_p;
+          hzx = hzh_p;
+        }
+        const T hxy = lh[p][0][w + 1][li], hzy = lh[p][2][w + 1][li];
         T a1 = (hzy - hzp[e]) * idy;  // Ex: dHz/dy - dHy/dz
         T a2 = (hy[e] - hyp[e]) * idz;
         T b1 = (hx[e] - hxp[e]) * idz;  // Ey: dHx/dz - dHz/dx
@@ -686,6 +781,11 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
       hyp[e] = hy[e];
       hzp[e] = hz[e];
     }
+    if constexpr (XS > 1) {
+      sr_exm = sr_ex;
+      hyh_p = hyh;
+      hzh_p = hzh;
+    }
   };
 
   PlaneIn pa, pb;
@@ -707,10 +807,11 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
     fzero(zero);
 #pragma unroll
     for (int e = 0; e < VEC; ++e) {
-      const int i = x0 + e;
-      const int ir = i + 1 < RW ? i + 1 : i;
-      T c1 = (lh[p][1][w][ir] - hyp[e]) * idx;
-      T c2 = (lh[p][0][w + 1][i] - hxp[e]) * idy;
+      const int li = x0 + e, i = gx0 + e;
+      const int ir = li + 1 < RW ? li + 1 : li;
+      const T hyx = (seg_r && li == RW - 1) ? hyh_p : lh[p][1][w][ir];
+      T c1 = (hyx - hyp[e]) * idx;
+      T c2 = (lh[p][0][w + 1][li] - hxp[e]) * idy;
       const bool ezok = i < nx - 1 && j < ny - 1;
       if (ezok && pml) {
         if (sys >= 0) c2 = fcpml_e<T>(g, z, c2, 1, 0, j, sys, i, j, kk);
@@ -739,13 +840,13 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
 // interior tiles fill the CUs the slab tiles leave idle at the end.  seg is
 // a multiple of 8, so a workgroup's XCD (blockIdx % 8) is the same in its
 // segment's numbering.
-template <typename T, int CK, int VEC, int W, int TY, bool ADE>
+template <typename T, int CK, int VEC, int W, int TY, int XS, bool ADE>
 __global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, const KFz z,
                                                                  int kchunk, int ntile_y,
                                                                  int nchunk, int n_out,
                                                                  int n_in, int seg, int order,
                                                                  TileBox bx) {
-  __shared__ FusedLds<T, VEC, W, TY> lds;
+  __shared__ FusedLds<T, VEC, W, TY, XS> lds;
   // dispersive instances: the box table copied to LDS once per workgroup,
   // so the per-row scans read LDS instead of issuing global loads whose
   // waits would also wait for the plane prefetch in flight
@@ -771,11 +872,11 @@ __global__ __launch_bounds__(64 * W * (TY + 1)) void k_yee_fused(const KGrid g, 
   }
   const int b = blockIdx.x;
   if (b < seg)
-    fused_tile<T, CK, VEC, W, TY, true, ADE>(g, z, kchunk, ntile_y, nchunk, n_out, order, bx,
-                                             b, lds, ade_tab, ade_cf);
+    fused_tile<T, CK, VEC, W, TY, XS, true, ADE>(g, z, kchunk, ntile_y, nchunk, n_out, order,
+                                                 bx, b, lds, ade_tab, ade_cf);
   else
-    fused_tile<T, CK, VEC, W, TY, false, ADE>(g, z, kchunk, ntile_y, nchunk, n_in, order, bx,
-                                              b - seg, lds, ade_tab, ade_cf);
+    fused_tile<T, CK, VEC, W, TY, XS, false, ADE>(g, z, kchunk, ntile_y, nchunk, n_in, order,
+                                                  bx, b - seg, lds, ade_tab, ade_cf);
 }
 
 template <typename T, int VEC, int W, int TY>
@@ -783,7 +884,7 @@ constexpr bool fused_fits() {
   return 64 * W * (TY + 1) <= 1024 && (size_t)9 * (TY + 1) * 64 * VEC * W * sizeof(T) <= 65536;
 }
 
-template <typename T, int CK, int VEC, int W, int TY>
+template <typename T, int CK, int VEC, int W, int TY, int XS>
 int launch_fused_one(const KGrid& g, const KFz& z, int kchunk, int order, hipStream_t s) {
   if constexpr (fused_fits<T, VEC, W, TY>()) {
     const int ntile_y = (g.ny + TY - 1) / TY;
@@ -802,19 +903,20 @@ int launch_fused_one(const KGrid& g, const KFz& z, int kchunk, int order, hipStr
       if (bx.kc1 > bx.kc0 && (bx.kc1 - 1) * kchunk + kchunk > zhi - 1) bx.kc1 = bx.kc0;
       if (bx.ty1 <= bx.ty0 || bx.kc1 <= bx.kc0) bx = TileBox{0, 0, 0, 0};
     }
-    const int n_in = (bx.ty1 - bx.ty0) * (bx.kc1 - bx.kc0);
-    const int n_out = ntile_y * nchunk - n_in;
+    // XS workgroups (row segments) per tile
+    const int n_in = (bx.ty1 - bx.ty0) * (bx.kc1 - bx.kc0) * XS;
+    const int n_out = ntile_y * nchunk * XS - n_in;
     const int seg = ((n_out + 7) / 8) * 8;
     const int nwg = seg + ((n_in + 7) / 8) * 8;
     // plans with dispersive boxes take an instance with the ADE update
     // inlined (rows that cut a box, checked per box); the others carry no
     // ADE code at all
     if (g.n_ade)
-      hipLaunchKernelGGL((k_yee_fused<T, CK, VEC, W, TY, true>), dim3(nwg),
+      hipLaunchKernelGGL((k_yee_fused<T, CK, VEC, W, TY, XS, true>), dim3(nwg),
                          dim3(64 * W * (TY + 1)), 0, s, g, z, kchunk, ntile_y, nchunk, n_out,
                          n_in, seg, order, bx);
     else
-      hipLaunchKernelGGL((k_yee_fused<T, CK, VEC, W, TY, false>), dim3(nwg),
+      hipLaunchKernelGGL((k_yee_fused<T, CK, VEC, W, TY, XS, false>), dim3(nwg),
                          dim3(64 * W * (TY + 1)), 0, s, g, z, kchunk, ntile_y, nchunk, n_out,
                          n_in, seg, order, bx);
     return 0;
@@ -823,19 +925,38 @@ int launch_fused_one(const KGrid& g, const KFz& z, int kchunk, int order, hipStr
   }
 }
 
-template <typename T, int CK, int VEC, int W>
+template <typename T, int CK, int VEC, int W, int XS = 1>
 int launch_fused_w(const KGrid& g, const KFz& z, int ty, int kchunk, int order,
                    hipStream_t s) {
-  if (ty == 2) return launch_fused_one<T, CK, VEC, W, 2>(g, z, kchunk, order, s);
-  if (ty == 3) return launch_fused_one<T, CK, VEC, W, 3>(g, z, kchunk, order, s);
-  if (ty == 4) return launch_fused_one<T, CK, VEC, W, 4>(g, z, kchunk, order, s);
+  if (ty == 2) return launch_fused_one<T, CK, VEC, W, 2, XS>(g, z, kchunk, order, s);
+  if (ty == 3) return launch_fused_one<T, CK, VEC, W, 3, XS>(g, z, kchunk, order, s);
+  if (ty == 4) return launch_fused_one<T, CK, VEC, W, 4, XS>(g, z, kchunk, order, s);
   return -1;
 }
 
-// (vec, w) pairs with 64*vec*w = row width 128, 256 or 512
+// Shapes built with rows in two segments (one cell per lane in segments of
+// 128 or 256 cells, or, FP32 only, two cells per lane in segments of 256;
+// not with per-cell Ca/Da/Db).
+constexpr bool fused_split_built(int esz, int ck, int vec, int w) {
+  return ck != PM_COEFF_FULL &&
+         ((vec == 1 && (w == 2 || w == 4)) || (esz == 4 && vec == 2 && w == 2));
+}
+
+// (vec, w) pairs with 64*vec*w = row width 128, 256 or 512 (xs = 1), or
+// half of it (xs = 2)
 template <typename T, int CK>
 int launch_fused_ck(const KGrid& g, const KFz& z, int vec, int w, int ty, int kchunk,
-                    int order, hipStream_t s) {
+                    int order, int xs, hipStream_t s) {
+  if (xs == 2) {
+    if constexpr (CK != PM_COEFF_FULL) {
+      if (vec == 1 && w == 2) return launch_fused_w<T, CK, 1, 2, 2>(g, z, ty, kchunk, order, s);
+      if (vec == 1 && w == 4) return launch_fused_w<T, CK, 1, 4, 2>(g, z, ty, kchunk, order, s);
+      if constexpr (sizeof(T) == 4)
+        if (vec == 2 && w == 2) return launch_fused_w<T, CK, 2, 2, 2>(g, z, ty, kchunk, order, s);
+    }
+    return -1;
+  }
+  if (xs != 1) return -1;
   if (vec == 1 && w == 2) return launch_fused_w<T, CK, 1, 2>(g, z, ty, kchunk, order, s);
   if (vec == 2 && w == 1) return launch_fused_w<T, CK, 2, 1>(g, z, ty, kchunk, order, s);
   if (vec == 1 && w == 4) return launch_fused_w<T, CK, 1, 4>(g, z, ty, kchunk, order, s);

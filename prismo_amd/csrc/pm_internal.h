@@ -178,11 +178,12 @@ void launch_yee_e(const KGrid& g, int precision, int coeff_kind, hipStream_t s);
 void launch_compat_flags2d(const KGrid& g, hipStream_t s);
 void launch_compat_h(const KGrid& g, int coeff_kind, hipStream_t s);
 void launch_compat_e(const KGrid& g, int coeff_kind, hipStream_t s);
+// xs: row segments per padded row (1, or 2: rows of 2 x 64 vec w cells).
 int launch_yee_fused_f32(const KGrid& g, int ck, const FusedBufs& fb, int vec, int w, int ty,
-                         int kchunk, int order, hipStream_t s);
+                         int kchunk, int order, int xs, hipStream_t s);
 int launch_yee_fused_f64(const KGrid& g, int ck, const FusedBufs& fb, int vec, int w, int ty,
-                         int kchunk, int order, hipStream_t s);
-bool fused_supported(int precision, int vec, int w, int ty);
+                         int kchunk, int order, int xs, hipStream_t s);
+bool fused_supported(int precision, int ck, int vec, int w, int ty, int xs);
 // Two Yee steps per launch (fused2_kernel.h), FP32 fields, Cb in FP32 or
 // FP16, no dispersive boxes; the sources of the first step (table row
 // src_row, n_src entries of the device table src) are added inside.
