@@ -296,6 +296,11 @@ int pm_set_tuning(pm_ctx* ctx, int fuse, int ty, int kchunk, int vec, int order)
  * autotuner; no counterpart in the reference. */
 int pm_set_row_split(pm_ctx* ctx, int segments);
 int pm_row_split_active(pm_ctx* ctx);
+/* Launch shape the next one-step fused launch takes: out[0..3] = cells per
+ * lane, waves per row segment, rows per workgroup, row segments (the
+ * tuning asked for, or what it falls back to); returns 1, or 0 (out all 0)
+ * when the step is not fused. */
+int pm_fused_shape(pm_ctx* ctx, int32_t* out);
 
 /* Steps per fused launch pm_run will use with the current plan and tuning
  * for a run without monitor sampling: 0 (unfused kernels), 1 or 2. */

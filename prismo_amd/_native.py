@@ -195,6 +195,7 @@ SIGNATURES = {
     "pm_sync": ([vp], C.c_int),
     "pm_set_row_split": ([vp, C.c_int], C.c_int),
     "pm_row_split_active": ([vp], C.c_int),
+    "pm_fused_shape": ([vp, C.POINTER(i32)], C.c_int),
     "pm_set_timing": ([vp, C.c_int], C.c_int),
     "pm_get_timing": ([vp, C.c_int, C.POINTER(f64), C.POINTER(i64)], C.c_int),
 }
@@ -292,6 +293,13 @@ class Context:
     def set_row_split(self, segments: int = 0):
         """Fused-kernel rows in 1 or 2 workgroups (0: automatic)."""
         self._check(self.lib.pm_set_row_split(self.h, int(segments)), "pm_set_row_split")
+
+    def fused_shape(self):
+        """(vec, w, ty, xs) of the next one-step fused launch, or None."""
+        out = (i32 * 4)()
+        if not self.lib.pm_fused_shape(self.h, out):
+            return None
+        return tuple(int(v) for v in out)
 
     def row_split(self) -> int:
         """Row segments the next fused step takes (0 when not fused)."""

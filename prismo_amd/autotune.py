@@ -22,7 +22,7 @@ _CACHE: dict = {}
 
 CANDIDATES = {
     "vec": (1, 2),
-    "ty": (2, 3, 4),
+    "ty": (2, 3, 4, 5, 7),
     "kchunk": (8, 12, 16, 24),
     "order": (2, 3),
 }
@@ -105,6 +105,7 @@ def tune(eng, steps: int = 6, warmup: int = 2, candidates: dict | None = None,
         shapes += [(2, 0, 4, kc, order, 1) for kc, order in product(c2["kchunk"], c2["order"])]
     saved = [t.clone() for t in _state_tensors(eng)]
     best = None
+    seen = set()
     try:
         for nstep, vec, ty, kc, order, xs in shapes:
             try:
@@ -112,8 +113,14 @@ def tune(eng, steps: int = 6, warmup: int = 2, candidates: dict | None = None,
                 eng.ctx.set_tuning(nstep, ty, kc, vec, order)
                 if eng.ctx.fused_steps() != nstep:
                     continue
-                if nstep == 1 and eng.ctx.row_split() != xs:
-                    continue  # the plan cannot take this many segments
+                if nstep == 1:
+                    # time each launch shape once, under the shape it runs
+                    # (a request the kernel cannot take falls back)
+                    shape = eng.ctx.fused_shape()
+                    if shape is None or (vec and shape[0] != vec) or shape[2] != ty \
+                            or shape[3] != xs or (shape, kc, order) in seen:
+                        continue
+                    seen.add((shape, kc, order))
                 # whole pairs of passes (4 steps) keep the two-step path on
                 run = lambda n: eng.ctx.run(0, -(-n // 4) * 4, 0)
                 run(warmup)

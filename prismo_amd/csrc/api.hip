@@ -1137,6 +1137,19 @@ int pm_set_row_split(pm_ctx* c, int segments) {
   return 0;
 }
 
+int pm_fused_shape(pm_ctx* c, int32_t* out) {
+  if (!c || !out) return PM_EINVAL;
+  out[0] = out[1] = out[2] = out[3] = 0;
+  if (!c->configured || !c->fused_ok || !c->fuse) return 0;
+  int vec = 0, w = 0, ty = 0, xs = 1;
+  if (!fused_shape(c, &vec, &w, &ty, &xs)) return 0;
+  out[0] = vec;
+  out[1] = w;
+  out[2] = ty;
+  out[3] = xs;
+  return 1;
+}
+
 int pm_row_split_active(pm_ctx* c) {
   if (!c || !c->configured || !c->fused_ok || !c->fuse) return 0;
   int vec = 0, w = 0, ty = 0, xs = 1;
@@ -1146,7 +1159,8 @@ int pm_row_split_active(pm_ctx* c) {
 
 int pm_set_tuning(pm_ctx* c, int fuse, int ty, int kchunk, int vec, int order) {
   if (!c) return PM_EINVAL;
-  if (ty < 2 || ty > 4) return fail(c, PM_EINVAL, "ty must be 2, 3 or 4");
+  if (ty < 2 || ty > 7 || ty == 6)
+    return fail(c, PM_EINVAL, "ty must be 2, 3, 4, 5 or 7 (5 and 7: one cell per lane, 128-cell rows)");
   if (kchunk < 1) return fail(c, PM_EINVAL, "kchunk must be >= 1");
   if (vec != 0 && vec != 1 && vec != 2 && vec != 4)
     return fail(c, PM_EINVAL, "vec must be 0 (auto), 1, 2 or 4");

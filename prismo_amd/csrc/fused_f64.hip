@@ -14,7 +14,8 @@ int launch_yee_fused_f64(const KGrid& g, int ck, const FusedBufs& fb, int vec, i
 
 bool fused_supported(int precision, int ck, int vec, int w, int ty, int xs) {
   const size_t t = precision == PM_FP64 ? 8 : 4;
-  if (ty < 2 || ty > 4) return false;
+  if (ty < 2 || ty > 7 || ty == 6) return false;
+  if (ty > 4 && !(vec == 1 && w == 2)) return false;
   if (xs == 2) {
     if (!fused_split_built((int)t, ck, vec, w)) return false;
   } else {

@@ -931,6 +931,11 @@ int launch_fused_w(const KGrid& g, const KFz& z, int ty, int kchunk, int order,
   if (ty == 2) return launch_fused_one<T, CK, VEC, W, 2, XS>(g, z, kchunk, order, s);
   if (ty == 3) return launch_fused_one<T, CK, VEC, W, 3, XS>(g, z, kchunk, order, s);
   if (ty == 4) return launch_fused_one<T, CK, VEC, W, 4, XS>(g, z, kchunk, order, s);
+  // taller tiles for 128-cell rows or segments (up to 16 waves)
+  if constexpr (VEC == 1 && W == 2) {
+    if (ty == 5) return launch_fused_one<T, CK, VEC, W, 5, XS>(g, z, kchunk, order, s);
+    if (ty == 7) return launch_fused_one<T, CK, VEC, W, 7, XS>(g, z, kchunk, order, s);
+  }
   return -1;
 }
 

@@ -81,3 +81,31 @@ def test_fp32_512_cell_rows_split_vs_whole(mi355x, vec, ty, ade):
     whole, _ = _run(shape, np.float32, "auto", 1, 40, ty=2, ade=ade)
     split, _ = _run(shape, np.float32, "auto", 2, 40, ty=ty, vec=vec, ade=ade)
     assert np.array_equal(_all(_fields(whole)), _all(_fields(split)))
+
+
+@pytest.mark.parametrize("dtype,storage,bar", [(np.float32, "fp16", 1e-5),
+                                               (np.float64, "auto", 1e-10)])
+@pytest.mark.parametrize("ty,kchunk,order", [(5, 7, 3), (7, 12, 2)])
+def test_tall_tiles_of_segments_vs_oracle(mi355x, dtype, storage, bar, ty, kchunk, order):
+    """5- and 7-row tiles of 128-cell segments (16 waves), which whole
+    256-cell rows cannot take, against O1 (fed the same FP16-rounded Cb in
+    FP32).  Whole rows are not a bitwise reference here: another tile height
+    moves cells between the kernel's slab and interior instances, which may
+    round differently."""
+    from oracle import compat as O0
+
+    steps = 61
+    sim, eps = _run((200, 30, 28), dtype, storage, 2, steps, ty, kchunk, order)
+    eng = sim.solver.updater.engine_for(sim.fields)
+    cb = None
+    if storage == "fp16":
+        _, Cb, _, _ = O0.coefficients(eps, np.ones_like(eps), np.zeros_like(eps),
+                                      np.zeros_like(eps), sim.dt)
+        scale = eng.cb_scale
+        cb = {c: (f(Cb) / scale).astype(np.float16).astype(np.float64) * scale
+              for c, f in (("Ex", O0._ex), ("Ey", O0._ey), ("Ez", O0._ez))}
+    o1 = _oracle(sim, eps, 5, cb=cb)
+    _drive(o1, sim, steps)
+    err = rel_l2(_all(_fields(sim)), _all(o1.F))
+    print(f"{np.dtype(dtype).name} ty {ty} segments vs O1: {err:.2e}")
+    assert err <= bar
