@@ -23,10 +23,11 @@ when it is present.
 The same line carries ``sweep``: 256 seeded directional couplers (2-D TM,
 200x120 .. 800x400 cells + 16-cell CPML, 4000 steps, 64-frequency ports)
 advanced together in FP32 with FP16 Cb by the batched kernel, timed end to
-end (build, run, |S21| analysis) and on the GPU, beside sequential plain
-FP64 runs of an evenly spaced sample of the same designs through
-Simulation.run; ``max_abs_s21_err`` is the largest |S21| difference
-between the two over that sample and all 64 frequencies.
+end (build, run, |S21| analysis) in pipelined groups (``designs_per_s``)
+and as one batch of all 256 (``designs_per_s_one_batch``), and on the GPU,
+beside sequential plain FP64 runs of an evenly spaced sample of 32 of the
+same designs through Simulation.run; ``max_abs_s21_err`` is the largest
+|S21| difference between the two over that sample and all 64 frequencies.
 
 ``strip_2d``: the 2-D TM strip (540 x 290 cells, FP64, 3000 steps) in the
 reference-arithmetic and textbook schemes on the GPU, beside the O0 NumPy
@@ -277,7 +278,7 @@ SWEEP_ALG_BYTES_PER_CELL = 26.0  # 2-D TM: Ez, Hx, Hy FP32 read + written, FP16 
 SWEEP_ALG_BYTES_PER_PML_CELL_AXIS = 16.0  # two FP32 psi read + written
 
 
-def sweep_bench(n_designs=256, steps=4000, seq_sample=12, group=64, first=16, roof_steps=512):
+def sweep_bench(n_designs=256, steps=4000, seq_sample=32, group=64, first=16):
     """Batched 256-coupler sweep through ParameterSweep's batched back end
     (FP32 fields, FP16 Cb, one launch per step over every design of a group
     of ``group``, the first group ``first`` designs so the GPU starts early,
@@ -285,9 +286,11 @@ def sweep_bench(n_designs=256, steps=4000, seq_sample=12, group=64, first=16, ro
     analyses while the GPU steps), timed end to end, against
     sequential plain FP64 runs of an evenly spaced sample of the same
     designs, after an untimed 64-step warm-up sweep of one group of each
-    size.  The roofline of
-    the batch kernel is taken on one launch over all 256 designs
-    (``roof_steps`` steps, HIP events)."""
+    size.  Then all designs in one batch, the shape the workload is stated
+    in (one launch per step over all 256, the step loop replayed as a HIP
+    graph), timed end to end (build, run, |S21|) after an untimed 64-step
+    run of that batch size (kernels loaded, launch shape tuned); the batch
+    kernel's roofline is taken on that run (HIP events)."""
     import numpy as np
     import torch
 
@@ -324,10 +327,21 @@ def sweep_bench(n_designs=256, steps=4000, seq_sample=12, group=64, first=16, ro
           f"(first group {first}, then groups of {group}, pipelined)", file=sys.stderr,
           flush=True)
 
-    # the batch kernel's roofline: one launch per step over all the designs
+    # all designs in one batch: one launch per step over all of them
+    sims = [W.coupler_sim(p, backend=W.host_backend()) for p in designs]
+    BatchEngine(sims, dtype=np.float32, coefficient_storage="fp16", **kw).run_steps(64)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
     sims = [W.coupler_sim(p, backend=W.host_backend()) for p in designs]
     eng = BatchEngine(sims, dtype=np.float32, coefficient_storage="fp16", **kw)
-    eng.run_steps(roof_steps, graph=True, timing=True)
+    eng.run_steps(steps, graph=True, timing=True)
+    one = [W.coupler_analyze(s) for s in sims]
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    one_rate = n_designs / (t3 - t2)
+    one_err = max(float(np.max(np.abs(one[k]["s21"] - res[k]["s21"]))) for k in range(n_designs))
+    print(f"[bench] sweep: {n_designs} designs in one batch end to end in {t3 - t2:.2f} s",
+          file=sys.stderr, flush=True)
     tuning = eng.tuning
     g_ms, g_n = eng.timing(2)
     e_ms, e_n = eng.timing(0)
@@ -342,7 +356,7 @@ def sweep_bench(n_designs=256, steps=4000, seq_sample=12, group=64, first=16, ro
         slab += 2 * t * ny + 2 * t * nx
     del eng, sims
     alg = SWEEP_ALG_BYTES_PER_CELL * cells + SWEEP_ALG_BYTES_PER_PML_CELL_AXIS * slab
-    step_ms = gpu_ms / roof_steps
+    step_ms = gpu_ms / steps
 
     idx = np.unique(np.linspace(0, n_designs - 1, seq_sample).round().astype(int))
     seq_t = 0.0
@@ -368,6 +382,8 @@ def sweep_bench(n_designs=256, steps=4000, seq_sample=12, group=64, first=16, ro
         "group": group,
         "first_group": first,
         "designs_per_s_gpu_only": round(n_designs / (step_ms * steps / 1e3), 3),
+        "designs_per_s_one_batch": round(one_rate, 3),
+        "one_batch_vs_groups_max_abs_s21_diff": one_err,
         "fp64_sequential_designs_per_s": round(seq_rate, 4),
         "fp64_sample": int(len(idx)),
         "speedup": round(rate / seq_rate, 2),
@@ -379,7 +395,7 @@ def sweep_bench(n_designs=256, steps=4000, seq_sample=12, group=64, first=16, ro
                      "achieved": round(alg / (step_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(alg / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                      "alg_bytes_per_step": alg, "step_ms": round(step_ms, 5),
-                     "steps_timed": roof_steps, "launch": tuning},
+                     "steps_timed": steps, "launch": tuning},
         "precision": "fp32 fields, fp16 Cb, fp32 compensated spectra; baseline fp64",
     }
 
