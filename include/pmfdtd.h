@@ -187,8 +187,9 @@ typedef struct pm_plan {
    * (gating flags and the device copy of the ADE box table) */
   void* scratch;
 
-  /* Optional second buffer set for the fused step (yee, 3-D, padded row
-   * sy in {128, 256, 512}).  With E2/H2 and
+  /* Optional second buffer set for the fused step (yee; 3-D with padded
+   * row sy in {128, 256, 512}, or 2-D TM without dispersive boxes or full
+   * coefficient arrays).  With E2/H2 and
    * psi_h2 (for every psi_h given) present, pm_run advances steps with one
    * fused H+E kernel that reads one set and writes the other; a run with an
    * odd step count does its first step in place, so every call returns with
@@ -274,8 +275,9 @@ int pm_run(pm_ctx* ctx, int64_t step0, int32_t nsteps, uint32_t flags);
  *    refused (PM_EINVAL). */
 int pm_dft_flush(pm_ctx* ctx, int64_t step_first, int32_t rows);
 
-/* Launch shape of the fused step: rows per workgroup (ty in {2, 3, 4}), z planes
- * per workgroup (kchunk >= 1), cells per lane (vec in {1, 2, 4}, 0 = auto)
+/* Launch shape of the fused step: rows per workgroup (ty in {2, 3, 4, 5, 7}),
+ * z planes per workgroup (kchunk >= 1; 2-D plans: rows per tile, default 3,
+ * ty, vec and order unused), cells per lane (vec in {1, 2, 4}, 0 = auto)
  * and tile order (0: y-tiles fastest, 1: z-chunks fastest, each XCD taking a
  * contiguous range; 2: y-tiles fastest in hardware round-robin; 3: as 2 in
  * groups of 64 tiles of which each XCD takes 8 neighbours); fuse = 0 forces

@@ -34,6 +34,9 @@ CANDIDATES2 = {
 }
 
 
+# 2-D one-pass step of one simulation: rows per tile
+CANDIDATES_2D = (1, 2, 3, 4, 8)
+
 # batched 2-D step: rows per tile (the plan's chunk), waves per workgroup,
 # XCD-grouped workgroup order
 BATCH_CANDIDATES = {
@@ -98,9 +101,13 @@ def tune(eng, steps: int = 6, warmup: int = 2, candidates: dict | None = None,
         return dict(best, fused=True, cached=True)
     cand = candidates or CANDIDATES
     splits = cand.get("xs", (1, 2))
-    shapes = [(1, vec, ty, kc, order, xs) for xs, vec, ty, kc, order in
-              product(splits, cand["vec"], cand["ty"], cand["kchunk"], cand["order"])]
-    if candidates is None or "two_step" in candidates:
+    if getattr(eng, "is_2d", False):
+        # the 2-D pass: rows per tile (kchunk) only
+        shapes = [(1, 0, 2, kc, 2, 1) for kc in cand.get("rows_2d", CANDIDATES_2D)]
+    else:
+        shapes = [(1, vec, ty, kc, order, xs) for xs, vec, ty, kc, order in
+                  product(splits, cand["vec"], cand["ty"], cand["kchunk"], cand["order"])]
+    if not getattr(eng, "is_2d", False) and (candidates is None or "two_step" in candidates):
         c2 = (candidates or {}).get("two_step", CANDIDATES2)
         shapes += [(2, 0, 4, kc, order, 1) for kc, order in product(c2["kchunk"], c2["order"])]
     saved = [t.clone() for t in _state_tensors(eng)]
@@ -117,7 +124,8 @@ def tune(eng, steps: int = 6, warmup: int = 2, candidates: dict | None = None,
                     # time each launch shape once, under the shape it runs
                     # (a request the kernel cannot take falls back)
                     shape = eng.ctx.fused_shape()
-                    if shape is None or (vec and shape[0] != vec) or shape[2] != ty \
+                    rows = kc if getattr(eng, "is_2d", False) else ty
+                    if shape is None or (vec and shape[0] != vec) or shape[2] != rows \
                             or shape[3] != xs or (shape, kc, order) in seen:
                         continue
                     seen.add((shape, kc, order))

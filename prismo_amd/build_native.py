@@ -41,6 +41,7 @@ UNITS = {
     "geom.hip": ["-ffp-contract=off"],
     "fused_f32.hip": [],
     "fused_f64.hip": [],
+    "tm2d.hip": [],
 }
 
 

@@ -350,10 +350,13 @@ class StepEngine:
 
     def _fill_fused(self, fused: bool):
         """Second buffer set for the fused ping-pong step (3-D yee, one x-row
-        per wave64: pitch / 64 in {2, 4, 8})."""
+        per wave64: pitch / 64 in {2, 4, 8}; 2-D TM yee: Ez, Hx, Hy)."""
         p = self.plan
         self.fused = False
-        if not fused or self.scheme != "yee" or self.is_2d:
+        if not fused or self.scheme != "yee":
+            return
+        if self.is_2d:
+            self._fill_fused_2d()
             return
         if self.sy not in (128, 256, 512):
             return
@@ -377,6 +380,26 @@ class StepEngine:
                     for q in range(2):
                         if src[a][q]:
                             dst[a][q] = self._alloc(shape, self.tdtype).data_ptr()
+        self.fused = True
+
+    def _fill_fused_2d(self):
+        """2-D TM (the yee scheme steps Ez, Hx, Hy in 2-D): a second Ez, Hx,
+        Hy and second H-side CPML memory (Hy in the x slabs, Hx in the y
+        slabs); lossy media and dispersive boxes keep the unfused kernels."""
+        p = self.plan
+        if p.n_ade or p.coeff_kind == N.COEFF_FULL:
+            return
+        nx, ny, _ = self.grid.dimensions
+        setb = self._alloc((3, 1, ny, self.sy), self.tdtype)
+        p.E2[2] = setb[0].data_ptr()
+        p.H2[0] = setb[1].data_ptr()
+        p.H2[1] = setb[2].data_ptr()
+        for a in range(2):
+            S = p.pml[2 * a] + p.pml[2 * a + 1]
+            shape = {0: (1, ny, S), 1: (1, S, nx)}[a]
+            for q in range(2):
+                if p.psi_h[a][q]:
+                    p.psi_h2[a][q] = self._alloc(shape, self.tdtype).data_ptr()
         self.fused = True
 
     def _fill_ade(self, regions):

@@ -31,6 +31,7 @@ struct pm_ctx {
   // tuning (0 auto, 1 whole rows, 2 two segments)
   bool split_ok = false;
   int tune_xs = 0;
+  bool kchunk_set = false;  // pm_set_tuning was called (the 2-D pass's rows)
   int parity = 0;
   // two steps per launch: plan eligible, and which E-side CPML memory set
   // holds the state (0 = plan.psi_e, 1 = plan.psi_e2)
@@ -341,10 +342,26 @@ int do_e(pm_ctx* c) {
 // Launch shape of the fused kernel: the tuned one, else (rows per workgroup
 // and then cells per lane relaxed) the first that fits the workgroup and LDS
 // limits.  False if none does.
+// Rows per tile of the 2-D pass: kchunk once pm_set_tuning has set it, else
+// 3.  A wave's rows run one after another (two rows of loads in flight), so
+// short tiles and many waves suit the small grids 2-D runs have: on a
+// 540 x 290 FP64 grid 2 and 3 rows per tile step in 7.3 us, 8 rows in
+// 11.5 us, the unfused H and E launches in 8.3 us.
+int rows2d(const pm_ctx* c) { return c->kchunk_set ? std::max(1, c->tune_kchunk) : 3; }
+
 // Rows in xs segments (pm_set_row_split: 1 or 2; 0 = whole rows when a
 // shape fits them, else two segments).
 bool fused_shape(const pm_ctx* c, int* vec, int* w, int* ty, int* xs) {
   const int sy = (int)c->plan.sy;  // padded row, 128 / 256 / 512
+  if (c->plan.ndim == 2) {
+    // the 2-D pass: one cell per lane, a wave per 64-cell strip, tiles of
+    // rows2d rows
+    *vec = 1;
+    *w = sy / 64;
+    *ty = rows2d(c);
+    *xs = 1;
+    return true;
+  }
   const int prec = c->plan.precision, ck = c->plan.coeff_kind;
   int opts[2] = {1, 2}, nopt = 2;
   if (c->tune_xs == 1 || c->tune_xs == 2) {
@@ -400,6 +417,26 @@ FusedBufs fused_bufs(const pm_plan& p, bool a_in) {
 int do_fused(pm_ctx* c) {
   Span sp(c, 3);
   const pm_plan& p = c->plan;
+  if (p.ndim == 2) {
+    const bool a_in = c->parity == 0;
+    Tm2dBufs b{};
+    b.Ei = a_in ? p.E[2] : p.E2[2];
+    b.Eo = a_in ? p.E2[2] : p.E[2];
+    for (int q = 0; q < 2; ++q) {
+      b.Hi[q] = a_in ? p.H[q] : p.H2[q];
+      b.Ho[q] = a_in ? p.H2[q] : p.H[q];
+    }
+    // H-side CPML memory: Hy in the x slabs (axis 0, first array), Hx in
+    // the y slabs (axis 1, second array)
+    b.psh_in[0] = a_in ? p.psi_h[0][0] : p.psi_h2[0][0];
+    b.psh_out[0] = a_in ? p.psi_h2[0][0] : p.psi_h[0][0];
+    b.psh_in[1] = a_in ? p.psi_h[1][1] : p.psi_h2[1][1];
+    b.psh_out[1] = a_in ? p.psi_h2[1][1] : p.psi_h[1][1];
+    if (launch_tm2d_fused(c->g, p.precision, p.coeff_kind, b, rows2d(c), c->stream))
+      return fail(c, PM_ENOTSUP, "fused 2-D step: plan not supported");
+    c->parity ^= 1;
+    return launch_status(c, "fused 2-D step launch");
+  }
   FusedBufs fb{};
   const bool a_in = c->parity == 0;
   for (int a = 0; a < 3; ++a) {
@@ -473,7 +510,9 @@ int do_fused2(pm_ctx* c, int64_t n, bool src) {
 
 bool fused2_eligible(const pm_ctx* c) {
   const pm_plan& p = c->plan;
-  if (!c->fused_ok || p.precision != PM_FP32 || p.n_ade || !fused2_grid_ok(c->g)) return false;
+  if (!c->fused_ok || p.ndim != 3 || p.precision != PM_FP32 || p.n_ade ||
+      !fused2_grid_ok(c->g))
+    return false;
   if (p.coeff_kind != PM_COEFF_CB && p.coeff_kind != PM_COEFF_CB_F16) return false;
   int vec, w;
   if (!fused2_shape(c, &vec, &w)) return false;
@@ -495,7 +534,21 @@ bool fused2_eligible(const pm_ctx* c) {
   return true;
 }
 
+// The 2-D TM one-pass step (tm2d.hip): Cb-only coefficients, no dispersive
+// boxes, a second set for Ez, Hx, Hy and for the H-side CPML memory.
+bool fused2d_eligible(const pm_plan& p) {
+  if (p.scheme != PM_SCHEME_YEE || p.ndim != 2 || p.n_ade) return false;
+  if (p.coeff_kind != PM_COEFF_CB && p.coeff_kind != PM_COEFF_CB_F16) return false;
+  if (!p.E2[2] || !p.H2[0] || !p.H2[1]) return false;
+  if ((uint64_t)p.sy * (uint64_t)p.ny >= (1ull << 31)) return false;
+  for (int a = 0; a < 2; ++a)
+    for (int q = 0; q < 2; ++q)
+      if (p.psi_h[a][q] && !p.psi_h2[a][q]) return false;
+  return true;
+}
+
 bool fused_eligible(const pm_plan& p) {
+  if (p.ndim == 2) return fused2d_eligible(p);
   if (p.scheme != PM_SCHEME_YEE || p.ndim != 3) return false;
   if (p.sy != 128 && p.sy != 256 && p.sy != 512) return false;
   if (p.nx > p.sy) return false;
@@ -764,9 +817,11 @@ int pm_configure(pm_ctx* c, const pm_plan* plan) {
     c->split_ok = p.coeff_kind != PM_COEFF_FULL && gxr < p.nx && gxr >= p.pml[0] &&
                   gxr < p.nx - p.pml[1];
   }
-  c->fused_ok = fused_eligible(p) &&
-                fused_addressable(c->g, fused_bufs(p, true), p.precision == PM_FP64 ? 8 : 4) &&
-                fused_addressable(c->g, fused_bufs(p, false), p.precision == PM_FP64 ? 8 : 4);
+  c->fused_ok = p.ndim == 2
+                    ? fused2d_eligible(p)
+                    : fused_eligible(p) &&
+                          fused_addressable(c->g, fused_bufs(p, true), p.precision == PM_FP64 ? 8 : 4) &&
+                          fused_addressable(c->g, fused_bufs(p, false), p.precision == PM_FP64 ? 8 : 4);
   c->parity = 0;
   c->pe_parity = 0;
   set_psi_e(c);
@@ -1174,6 +1229,7 @@ int pm_set_tuning(pm_ctx* c, int fuse, int ty, int kchunk, int vec, int order) {
   c->fuse = fuse;
   c->tune_ty = ty;
   c->tune_kchunk = kchunk;
+  c->kchunk_set = true;
   c->tune_vec = vec;
   c->tune_order = order;
   return 0;

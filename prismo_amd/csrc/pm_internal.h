@@ -178,6 +178,19 @@ void launch_yee_e(const KGrid& g, int precision, int coeff_kind, hipStream_t s);
 void launch_compat_flags2d(const KGrid& g, hipStream_t s);
 void launch_compat_h(const KGrid& g, int coeff_kind, hipStream_t s);
 void launch_compat_e(const KGrid& g, int coeff_kind, hipStream_t s);
+// One fused 2-D TM step (tm2d.hip): Ez, Hx, Hy of set "i" -> set "o", the
+// H-side CPML memory of the x (Hy) and y (Hx) slabs likewise, the E-side
+// memory of the plan updated in place; rows per tile `chunk`.
+struct Tm2dBufs {
+  const void* Ei;
+  const void* Hi[2];
+  void* Eo;
+  void* Ho[2];
+  const void* psh_in[2];
+  void* psh_out[2];
+};
+int launch_tm2d_fused(const KGrid& g, int precision, int ck, const Tm2dBufs& b, int chunk,
+                      hipStream_t s);
 // xs: row segments per padded row (1, or 2: rows of 2 x 64 vec w cells).
 int launch_yee_fused_f32(const KGrid& g, int ck, const FusedBufs& fb, int vec, int w, int ty,
                          int kchunk, int order, int xs, hipStream_t s);
