@@ -76,6 +76,28 @@ __device__ __forceinline__ uint32_t fslab_off(const KGrid& g, int a, int s, int 
   return (uint32_t)i + (uint32_t)g.nx * ((uint32_t)j + (uint32_t)g.ny * (uint32_t)s);
 }
 
+// Row segments of FP64 plans take their values from beyond the segment ends
+// as scalar loads (false) or as one lane-spread vector load per plane
+// (true; two more vector registers per plane set, which the FP64 instances
+// pay in spills).
+#ifndef PM_FUSED_LANE_HALO_F64
+#define PM_FUSED_LANE_HALO_F64 0
+#endif
+constexpr bool kFusedLaneHaloF64 = PM_FUSED_LANE_HALO_F64 != 0;
+
+// Lane l's value of v, for every lane (v_readlane; FP64 as two halves).
+template <typename T>
+__device__ __forceinline__ T lane_bcast(T v, int l) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+  } else {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+    return __builtin_bit_cast(T, ((unsigned long long)hi << 32) | lo);
+  }
+}
+
 // E-side CPML term, in place (E is computed on own cells only).
 template <typename T>
 __device__ __forceinline__ T fcpml_e(const KGrid& g, const KFz& z, T d, int a, int q, int p,
@@ -287,7 +309,7 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
   // this segment has a segment to its left / right
   const bool seg_l = XS > 1 && xs > 0;
   const bool seg_r = XS > 1 && xs + 1 < XS;
-  const bool lane_l = x0 == 0, lane_r = x0 == RW - VEC;
+  const bool lane_r = x0 == RW - VEC;
   const int j = ty * TY + w;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const bool row_ok = j < ny;
@@ -377,11 +399,52 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
     }
   };
 
+  // Row segments: the values a segment needs from beyond its ends, one per
+  // lane of every wave, loaded with the plane (lanes 0-1: Ey, Ez left of
+  // the segment; 2-6: Ex, Ey, Ez, Hy, Hz right of it; 7: Ex right of it on
+  // the row above the tile's first row; 8-9, slab tiles: the y and z H-side
+  // CPML memory right of it; other lanes repeat lane 2).  Scalar loads
+  // would cost no vector registers, but their counter is the one LDS
+  // accesses use and they return out of order, so the first LDS wait of
+  // every plane would wait for them.  A segment at the row's end reads
+  // column 0 or nx-1 (unused).
+  constexpr bool HL = XS > 1 && (sizeof(T) == 4 || kFusedLaneHaloF64);
+  const int hlane = tid & 63;
+  const char* hbase = z.bi;
+  uint32_t hoff0 = 0, hstep = 0;
+  if constexpr (HL) {
+    const uint32_t esz = (uint32_t)sizeof(T);
+    const uint32_t rcol0 = (uint32_t)(seg_r ? min((xs + 1) * RW, nx - 1) : 0);
+    const uint32_t lcol0 = (uint32_t)(seg_l ? xs * RW - 1 : 0);
+    const uint32_t rowr = (uint32_t)jr * sy + rcol0;
+    hstep = sz * esz;
+    if (hlane < 2) {
+      hoff0 = (uint32_t)(hlane + 1) * z.cs + ((uint32_t)jr * sy + lcol0) * esz;
+    } else if (hlane < 7) {
+      const uint32_t c = hlane == 2 ? 0u : hlane == 3 ? 1u : hlane == 4 ? 2u : hlane == 5 ? 4u : 5u;
+      hoff0 = c * z.cs + rowr * esz;
+    } else if (hlane == 7) {
+      hoff0 = ((uint32_t)(first_row ? j - 1 : jr) * sy + rcol0) * esz;
+    } else if (YZ && hlane == 8) {
+      hbase = z.bc;
+      const uint32_t Sy = (uint32_t)(g.pml[2] + g.pml[3]);
+      hoff0 = z.phi[1] + fslab_off(g, 1, max(sys, 0), (int)rcol0, jr, 0) * esz;
+      hstep = (uint32_t)nx * Sy * esz;
+    } else if (YZ && hlane == 9) {
+      hbase = z.bc;
+      hoff0 = z.phi[2] + z.ps[2] + (rcol0 + (uint32_t)nx * (uint32_t)jr) * esz;
+      hstep = 0;
+    } else {
+      hoff0 = rowr * esz;
+    }
+  }
+
   struct PlaneIn {
     T ex[VEC], ey[VEC], ez[VEC];  // E(n) of the plane
     T hx[VEC], hy[VEC], hz[VEC];  // H(n-1/2)
     T exu[VEC], ezu[VEC];         // row j-1 (first row of the tile only)
     T q[6][VEC];                  // H-side CPML memory
+    T hl;                         // row segments: this lane's value from beyond the ends
   };
   auto load_plane = [&](int kq, PlaneIn& P) __attribute__((always_inline)) {
     const uint32_t o = rowoff + (uint32_t)kq * sz;
@@ -397,6 +460,15 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
     fld(P.exu, zp<T>(z.bi, 0u, u));
     fld(P.ezu, zp<T>(z.bi, 2u * z.cs, u));
     load_psi(z.phi, kq, P.q);
+    if constexpr (HL) {
+      uint32_t ho = hoff0 + (uint32_t)kq * hstep;
+      if constexpr (YZ) {
+        // z slab memory: slab plane index of kq (0 outside the z layers)
+        const int skh = max(fslab(kq, nz, g.pml[4], g.pml[5]), 0);
+        if (hlane == 9) ho += (uint32_t)nx * (uint32_t)ny * (uint32_t)skh * (uint32_t)sizeof(T);
+      }
+      P.hl = *(gptr<const T>)(hbase + ho);
+    }
   };
 
   T exm[VEC], eym[VEC], ezm[VEC];  // E(n) of plane k-1
@@ -469,7 +541,20 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
     //     column 0 or nx-1, unused)
     T sl_ey = 0, sl_ez = 0;
     T sr_ex = 0, sr_ey = 0, sr_ez = 0, sr_hy = 0, sr_hz = 0, sr_exu = 0, sr_q2 = 0, sr_q5 = 0;
-    if constexpr (XS > 1) {
+    if constexpr (HL) {
+      sl_ey = lane_bcast(cur.hl, 0);
+      sl_ez = lane_bcast(cur.hl, 1);
+      sr_ex = lane_bcast(cur.hl, 2);
+      sr_ey = lane_bcast(cur.hl, 3);
+      sr_ez = lane_bcast(cur.hl, 4);
+      sr_hy = lane_bcast(cur.hl, 5);
+      sr_hz = lane_bcast(cur.hl, 6);
+      sr_exu = lane_bcast(cur.hl, 7);
+      if constexpr (YZ) {
+        sr_q2 = lane_bcast(cur.hl, 8);
+        sr_q5 = lane_bcast(cur.hl, 9);
+      }
+    } else if constexpr (XS > 1) {
       const uint32_t lo = (uint32_t)jr * sy + lcol + (uint32_t)k * sz;
       sl_ey = sld(z.bi, z.cs, lo);
       sl_ez = sld(z.bi, 2u * z.cs, lo);
