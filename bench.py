@@ -532,12 +532,15 @@ def waveguide_alg_bytes(sim, esz: int, csz: int) -> int:
     return (12 * esz + 3 * csz) * cells + 8 * esz * slab + ade
 
 
-def waveguide_bench(steps=None):
+def waveguide_bench(steps=None, launch32=None, launch64=None):
     """Dispersive 3-D waveguide (480 x 240 x 160 cells, Lorentz Si core,
     Drude pad, grid-mode source, 64-frequency DFT plane): FP32 fields with
     FP32 and with FP16 Cb, against FP64 over the full run; relative L2 of
     the plane's spectra and of the flux, and the step kernel's roofline
-    (algorithmic bytes per step over its HIP-event time)."""
+    (algorithmic bytes per step over its HIP-event time).  launch32 /
+    launch64 = (vec, ty, kchunk, order, row segments) pin the fused step's
+    launch shape of the FP32 legs / the FP64 leg (for counter runs, whose
+    serialised kernels would skew the tuner); by default each leg is tuned."""
     import numpy as np
     import torch
 
@@ -548,8 +551,16 @@ def waveguide_bench(steps=None):
     for name, dt, storage in (("fp32", np.float32, "auto"), ("fp32_fp16cb", np.float32, "fp16"),
                               ("fp64", np.float64, "auto")):
         sim, mon, mode, _ = W.waveguide_3d(dtype=dt, coefficient_storage=storage)
-        tuned = sim.autotune()
         eng = sim.solver.updater.engine_for(sim.fields)
+        pin = launch64 if dt == np.float64 else launch32
+        if pin:
+            vec, ty, kc, order, xs = pin
+            eng.ctx.set_row_split(xs)
+            eng.ctx.set_tuning(1, ty, kc, vec, order)
+            tuned = {"vec": vec, "ty": ty, "kchunk": kc, "order": order, "xs": xs,
+                     "fused": bool(eng.ctx.fused_active())}
+        else:
+            tuned = sim.autotune()
         sim.run_steps(2)
         sim.prepare(steps - 2)
         torch.cuda.synchronize()
@@ -624,6 +635,10 @@ def main():
     ap.add_argument("--no-waveguide", action="store_true")
     ap.add_argument("--waveguide-only", action="store_true")
     ap.add_argument("--waveguide-steps", type=int, default=None)
+    ap.add_argument("--waveguide-launch", type=str, default=None,
+                    help="vec,ty,kchunk,order,xs of the FP32 waveguide legs (default: tuned)")
+    ap.add_argument("--waveguide-launch64", type=str, default=None,
+                    help="vec,ty,kchunk,order,xs of the FP64 waveguide leg (default: tuned)")
     ap.add_argument("--fp64-only", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -658,8 +673,12 @@ def main():
                                                  / cpu_sweep["designs_per_s"], 1)
         print(json.dumps({"sweep": sweep}), flush=True)
         return
+    pin = lambda t: tuple(int(x) for x in t.split(",")) if t else None
     if args.waveguide_only:
-        print(json.dumps({"waveguide_3d": waveguide_bench(args.waveguide_steps)}), flush=True)
+        print(json.dumps({"waveguide_3d": waveguide_bench(args.waveguide_steps,
+                                                          pin(args.waveguide_launch),
+                                                          pin(args.waveguide_launch64))}),
+              flush=True)
         return
     if args.fp64_only:
         print(json.dumps({"voxel_fp64": voxel_fp64(args.n, args.steps, args.warmup, None)}),
