@@ -529,6 +529,26 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
       }
     }
   }
+  // the boxes that hold a cell of this wave (over all its lanes) and the
+  // union of their plane ranges: waves with no cell in a box, and planes
+  // outside that union, skip the per-plane box code (a filter only: where
+  // it passes, each lane still finds its box per plane).  FP64 instances
+  // filter by the row's boxes and the union of all boxes' planes instead
+  // (at 168 VGPRs the per-wave filter costs them spills).
+  int wk0 = g.ade_k0, wk1 = (rc0 | rc1 | rc2) != 0 ? g.ade_k1 : 0;
+  if constexpr (ADE && sizeof(T) == 4) {
+    uint32_t wb = 0;
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) wb |= xm[e];
+    for (int m = 32; m >= 1; m >>= 1) wb |= (uint32_t)__shfl_xor((int)wb, m);
+    wb = (uint32_t)__builtin_amdgcn_readfirstlane((int)wb);
+    wk0 = nz;
+    for (int t = 0; t < g.n_ade; ++t)
+      if ((wb >> t) & 1u) {
+        wk0 = min(wk0, tab[t].k0);
+        wk1 = max(wk1, tab[t].k1);
+      }
+  }
 
   auto step = [&](const int k, const PlaneIn& cur, PlaneIn& nxt) __attribute__((always_inline)) {
     const uint32_t off = rowoff + (uint32_t)k * sz;
@@ -597,8 +617,7 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
     //      does not wait for them.  No loop here or in (6) issues a memory
     //      access: with a loop-dependent count of accesses in flight the
     //      compiler would wait for all of them, the plane prefetch included.
-    const bool ade_row =
-        ADE && do_e && (rc0 | rc1 | rc2) != 0 && kk >= g.ade_k0 && kk < g.ade_k1;
+    const bool ade_row = ADE && do_e && kk >= wk0 && kk < wk1;
     // box number + 1 of (component c, cell e) in 4 bits at 4 (c VEC + e);
     // 0: no box
     ABits abits = 0;
