@@ -3,7 +3,7 @@ the unfused 2-D kernels and against O1.
 
 Grids with CPML on the four sides (one whose strip boundary at x = 128
 falls inside the x layer, so the recomputed Hy right of a strip takes its
-CPML term), without CPML, FP64 and FP32 with FP16 Cb; tiles of 2 to 32
+CPML term), without CPML, FP64 and FP32 with FP16 Cb; tiles of 1 to 32
 rows; odd run lengths (the first step in place) and runs split into
 pieces; a plane-wave line source and a spectral line monitor.
 """
