@@ -76,15 +76,6 @@ __device__ __forceinline__ uint32_t fslab_off(const KGrid& g, int a, int s, int 
   return (uint32_t)i + (uint32_t)g.nx * ((uint32_t)j + (uint32_t)g.ny * (uint32_t)s);
 }
 
-// Row segments of FP64 plans take their values from beyond the segment ends
-// as scalar loads (false) or as one lane-spread vector load per plane
-// (true; two more vector registers per plane set, which the FP64 instances
-// pay in spills).
-#ifndef PM_FUSED_LANE_HALO_F64
-#define PM_FUSED_LANE_HALO_F64 0
-#endif
-constexpr bool kFusedLaneHaloF64 = PM_FUSED_LANE_HALO_F64 != 0;
-
 // Lane l's value of v, for every lane (v_readlane; FP64 as two halves).
 template <typename T>
 __device__ __forceinline__ T lane_bcast(T v, int l) {
@@ -407,8 +398,10 @@ __device__ __forceinline__ void fused_tile(const KGrid& g, const KFz& z, int kch
   // would cost no vector registers, but their counter is the one LDS
   // accesses use and they return out of order, so the first LDS wait of
   // every plane would wait for them.  A segment at the row's end reads
-  // column 0 or nx-1 (unused).
-  constexpr bool HL = XS > 1 && (sizeof(T) == 4 || kFusedLaneHaloF64);
+  // column 0 or nx-1 (unused).  FP64 instances keep the scalar loads: at
+  // 168 VGPRs the lane-spread load's two registers per plane set cost them
+  // spills.
+  constexpr bool HL = XS > 1 && sizeof(T) == 4;
   const int hlane = tid & 63;
   const char* hbase = z.bi;
   uint32_t hoff0 = 0, hstep = 0;
