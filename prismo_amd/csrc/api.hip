@@ -32,6 +32,9 @@ struct pm_ctx {
   bool split_ok = false;
   int tune_xs = 0;
   bool kchunk_set = false;  // pm_set_tuning was called (the 2-D pass's rows)
+  // 2-D compat gate flags: the set the next H update fills (0 / 1) and the
+  // set the E update of the current step reads
+  int flag_set = 0, flag_set_e = 0;
   int parity = 0;
   // two steps per launch: plan eligible, and which E-side CPML memory set
   // holds the state (0 = plan.psi_e, 1 = plan.psi_e2)
@@ -53,7 +56,8 @@ struct pm_ctx {
   size_t pool_used = 0;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> spans[7];
   // graph replay (PM_RUN_GRAPH): one captured block of steps per starting
-  // buffer set, with the run flags it was captured for
+  // buffer set (fused steps) or gate flag set (2-D compat steps), with the
+  // run flags it was captured for
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   uint32_t graph_key[2] = {0, 0};
 };
@@ -316,12 +320,29 @@ int launch_status(pm_ctx* c, const char* what) {
   return hip_check(c, hipGetLastError(), what);
 }
 
+// The gate flag sets of a 2-D compat step: `set` filled before the H
+// update and read by it and the E update, the other one cleared by the H
+// kernel for the next step.
+KGrid flag_grid(const pm_ctx* c, int set) {
+  KGrid g = c->g;
+  g.flags = c->g.flags + 8 * set;
+  g.flags_next = c->g.flags + 8 * (set ^ 1);
+  return g;
+}
+
 int do_h(pm_ctx* c) {
   Span sp(c, 0);
   const pm_plan& p = c->plan;
   if (p.scheme == PM_SCHEME_COMPAT) {
-    if (p.ndim == 2) launch_compat_flags2d(c->g, c->stream);
-    launch_compat_h(c->g, p.coeff_kind, c->stream);
+    if (p.ndim == 2) {
+      const KGrid g = flag_grid(c, c->flag_set);
+      launch_compat_flags2d(g, c->stream);
+      launch_compat_h(g, p.coeff_kind, c->stream);
+      c->flag_set_e = c->flag_set;
+      c->flag_set ^= 1;
+    } else {
+      launch_compat_h(c->g, p.coeff_kind, c->stream);
+    }
   } else {
     launch_yee_h(c->g, p.precision, p.coeff_kind, c->stream);
   }
@@ -332,7 +353,7 @@ int do_e(pm_ctx* c) {
   Span sp(c, 1);
   const pm_plan& p = c->plan;
   if (p.scheme == PM_SCHEME_COMPAT)
-    launch_compat_e(c->g, p.coeff_kind, c->stream);
+    launch_compat_e(p.ndim == 2 ? flag_grid(c, c->flag_set_e) : c->g, p.coeff_kind, c->stream);
   else
     launch_yee_e(c->g, p.precision, p.coeff_kind, c->stream);
   return launch_status(c, "E update launch");
@@ -830,6 +851,13 @@ int pm_configure(pm_ctx* c, const pm_plan* plan) {
   for (int s = 0; s < x.n_src; ++s)
     if (x.src[s].kind != 0) c->src_kind0 = false;
   c->pend_n = 0;
+  // both gate flag sets start cleared (2-D compat)
+  c->flag_set = c->flag_set_e = 0;
+  if (p.scheme == PM_SCHEME_COMPAT && p.ndim == 2) {
+    launch_compat_flags_clear(c->g.flags, c->stream);
+    int r = launch_status(c, "flag clear launch");
+    if (r) return r;
+  }
   c->configured = true;
   return 0;
 }
@@ -949,12 +977,13 @@ int eager_step(pm_ctx* c, int64_t n, bool in_place, bool fused, bool src, bool s
   return 0;
 }
 
-// Capture one block of G steps that starts in buffer set `par`, on the
-// second stream: per step the fields, the sources (table row from the step
+// Capture one block of G steps that starts in buffer set `par` into graph
+// slot `slot`, on the second stream: per step the fields, the sources (table row from the step
 // clock, which holds the block's first step) and the samples (row k of X);
 // then the block's spectra (phasor columns from the clock) and the clock
 // advanced by G.  Kernels only: nothing in the graph clears memory.
-int capture_block(pm_ctx* c, int par, int32_t G, bool fused, bool src, bool smp, bool dft) {
+int capture_block(pm_ctx* c, int par, int slot, int32_t G, bool fused, bool src, bool smp,
+                  bool dft) {
   const pm_plan& p = c->plan;
   int64_t* clk = step_clock(c);
   hipStream_t s0 = c->stream;
@@ -1007,7 +1036,7 @@ int capture_block(pm_ctx* c, int par, int32_t G, bool fused, bool src, bool smp,
   c->parity = par0;
   c->span_on = span0;
   if (!r)
-    r = hip_check(c, hipGraphInstantiate(&c->graph[par], g, nullptr, nullptr, 0),
+    r = hip_check(c, hipGraphInstantiate(&c->graph[slot], g, nullptr, nullptr, 0),
                   "graph instantiate");
   if (g) (void)hipGraphDestroy(g);
   return r;
@@ -1017,7 +1046,10 @@ int capture_block(pm_ctx* c, int par, int32_t G, bool fused, bool src, bool smp,
 // stream, ordered after the caller's stream's work and before its next.
 int replay_blocks(pm_ctx* c, int64_t n0, int32_t nrep, int32_t G, bool fused, bool src,
                   bool smp, bool dft) {
-  const int par = c->parity;
+  // graphs per starting buffer set (fused steps) or per starting gate flag
+  // set (2-D compat steps; blocks hold an even number of steps, so a block
+  // ends on the set it starts with)
+  const int par = fused ? c->parity : c->flag_set;
   const uint32_t key = 1u | (fused ? 2u : 0u) | (src ? 4u : 0u) | (smp ? 8u : 0u) |
                        (dft ? 16u : 0u) | ((uint32_t)G << 5);
   int r;
@@ -1026,7 +1058,7 @@ int replay_blocks(pm_ctx* c, int64_t n0, int32_t nrep, int32_t G, bool fused, bo
       (void)hipGraphExecDestroy(c->graph[par]);
       c->graph[par] = nullptr;
     }
-    if ((r = capture_block(c, par, G, fused, src, smp, dft))) return r;
+    if ((r = capture_block(c, c->parity, par, G, fused, src, smp, dft))) return r;
     c->graph_key[par] = key;
   }
   launch_clock(step_clock(c), n0, false, c->stream);

@@ -32,10 +32,12 @@
 namespace pm {
 
 // flags[2a] = some element non-zero and not NaN, flags[2a+1] = some NaN,
-// for a = 0: Ez, 1: Ex, 2: Ey, 3: Hz (2-D plane, padded layout), cleared
-// and recomputed before every H update.
+// for a = 0: Ez, 1: Ex, 2: Ey, 3: Hz (2-D plane, padded layout), computed
+// before every H update into one of two sets of eight words that steps
+// alternate: the H kernel of a step clears the other set, which the next
+// step's flags kernel fills (both sets start cleared at configure).
 __global__ void k_flags_clear(int* f) {
-  if (threadIdx.x < 8) f[threadIdx.x] = 0;
+  if (threadIdx.x < 16) f[threadIdx.x] = 0;
 }
 
 constexpr int kFlagRows = 4;
@@ -98,6 +100,9 @@ template <int CK>
 __global__ __launch_bounds__(kBX* kBY) void k_compat_h2d(const KGrid g) {
   const int i = blockIdx.x * kBX + threadIdx.x;
   const int j = blockIdx.y * kBY + threadIdx.y;
+  // the next step's flag set (read by nothing in this step)
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.y == 0 && threadIdx.x < 8)
+    g.flags_next[threadIdx.x] = 0;
   if (i >= g.nx || j >= g.ny) return;
   const int* f = g.flags;
   const bool g_tm = f[0] && !f[1];
@@ -220,8 +225,11 @@ __global__ __launch_bounds__(kBX* kBY) void k_compat_e3d(const KGrid g) {
   }
 }
 
+void launch_compat_flags_clear(int* flags, hipStream_t s) {
+  hipLaunchKernelGGL(k_flags_clear, dim3(1), dim3(64), 0, s, flags);
+}
+
 void launch_compat_flags2d(const KGrid& g, hipStream_t s) {
-  hipLaunchKernelGGL(k_flags_clear, dim3(1), dim3(64), 0, s, g.flags);
   const dim3 grid((g.nx + 255) / 256, (g.ny + kFlagRows - 1) / kFlagRows);
   hipLaunchKernelGGL(k_flags2d, grid, dim3(256), 0, s, g);
 }

@@ -73,7 +73,8 @@ struct KGrid {
   int n_ade;
   int ade_j0, ade_j1, ade_k0, ade_k1;  // union of the boxes' j and k ranges
   const KAde* ade;  // device copy of the ADE boxes (in scratch)
-  int* flags;       // compat gating flags (scratch)
+  int* flags;       // compat gating flags of this step (scratch)
+  int* flags_next;  // the other set, cleared by the 2-D compat H kernel
 };
 
 // scratch layout (caller-owned, PM_SCRATCH_BYTES)
@@ -176,6 +177,7 @@ struct FusedBufs {
 void launch_yee_h(const KGrid& g, int precision, int coeff_kind, hipStream_t s);
 void launch_yee_e(const KGrid& g, int precision, int coeff_kind, hipStream_t s);
 void launch_compat_flags2d(const KGrid& g, hipStream_t s);
+void launch_compat_flags_clear(int* flags, hipStream_t s);
 void launch_compat_h(const KGrid& g, int coeff_kind, hipStream_t s);
 void launch_compat_e(const KGrid& g, int coeff_kind, hipStream_t s);
 // One fused 2-D TM step (tm2d.hip): Ez, Hx, Hy of set "i" -> set "o", the
