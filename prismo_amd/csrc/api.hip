@@ -559,7 +559,9 @@ bool fused2_eligible(const pm_ctx* c) {
 // boxes, a second set for Ez, Hx, Hy and for the H-side CPML memory.
 bool fused2d_eligible(const pm_plan& p) {
   if (p.scheme != PM_SCHEME_YEE || p.ndim != 2 || p.n_ade) return false;
-  if (p.coeff_kind != PM_COEFF_CB && p.coeff_kind != PM_COEFF_CB_F16) return false;
+  if (p.coeff_kind != PM_COEFF_CB &&
+      !(p.coeff_kind == PM_COEFF_CB_F16 && p.precision == PM_FP32))
+    return false;
   if (!p.E2[2] || !p.H2[0] || !p.H2[1]) return false;
   if ((uint64_t)p.sy * (uint64_t)p.ny >= (1ull << 31)) return false;
   for (int a = 0; a < 2; ++a)
