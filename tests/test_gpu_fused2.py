@@ -4,7 +4,8 @@ against the one-step fused kernel and against O1.
 The two-step pass computes every cell with the same per-step arithmetic as
 the one-step kernel, so over a whole run the fields agree to FP32 rounding
 of the instruction order (<= 1e-6 relative L2), and both are within the
-north star's 1e-5 of O1 fed the same (FP16-rounded) coefficients.  Cases:
+FP32 accuracy bar of 1e-5 against O1 fed the same (FP16-rounded)
+coefficients.  Cases:
 padded rows of 128 and 256 cells, CPML on all faces with interior and slab
 tiles, z-chunks of several heights and both tile orders, run lengths that
 leave one to three steps before the pairs of passes, two point sources (one

@@ -260,7 +260,7 @@ def test_yee_3d_cpml(mi355x, dtype):
     o1 = _yee_oracle(sim, eps, t)
     ref = _run_o1(o1, sim, steps, mon)
     # FP32 fields against the FP64 oracle fed the same FP64 coefficients:
-    # the north star's 1e-5 bar on every component and every spectrum
+    # the FP32 accuracy bar, 1e-5, on every component and every spectrum
     tol = 1e-10 if dtype == "float64" else 1e-5
     comps = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
     errs = {c: rel_l2(sim.fields.to_numpy(c), o1.F[c]) for c in comps}
