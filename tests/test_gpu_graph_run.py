@@ -4,7 +4,8 @@ step: bitwise equal fields, spectra and time-domain samples, and a graph
 run against O1.  (Small grids take graph replays by default, so the 2-D
 parity tests against O0 and O1 run through them too.)
 
-Cases: the 2-D strip in the reference arithmetic (compat, FP64) and in the
+Cases: the 2-D strip in the reference arithmetic (compat, FP64; also with
+every block starting on the second of its two gate flag sets) and in the
 textbook scheme with CPML; a small 3-D grid with CPML in FP32 (fused
 ping-pong kernel) with two dipoles, a plane-wave source and a spectral
 monitor; odd run lengths (the in-place first step), runs split at steps that
@@ -50,6 +51,27 @@ def test_strip_graph_equals_direct(mi355x, scheme):
         out.append((_fields(sim), _spec(mon), sim.current_time))
     (fa, sa, ta), (fb, sb, tb) = out
     assert ta == tb
+    for c in ("Ez", "Hx", "Hy"):
+        assert np.array_equal(fa[c], fb[c]), c
+    assert np.array_equal(sa, sb)
+
+
+def test_compat_strip_graph_blocks_on_odd_flag_set(mi355x):
+    """2-D compat steps alternate between two sets of gate flags; an extra
+    H half-step before the run makes every graph block start on the second
+    set (the graphs are kept per starting set).  Graph replay equals direct
+    launches bitwise."""
+    out = []
+    for graph in (False, True):
+        sim, mon = _strip("compat", 700)
+        sim.graph = graph
+        sim.run_steps(40)
+        up = sim.solver.updater
+        up.update_magnetic_fields(sim.fields)   # one H without its E
+        for n in (300, 173):
+            sim.run_steps(n)
+        out.append((_fields(sim), _spec(mon)))
+    (fa, sa), (fb, sb) = out
     for c in ("Ez", "Hx", "Hy"):
         assert np.array_equal(fa[c], fb[c]), c
     assert np.array_equal(sa, sb)
