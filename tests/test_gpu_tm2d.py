@@ -3,7 +3,7 @@ the unfused 2-D kernels and against O1.
 
 Grids with CPML on the four sides (one whose strip boundary at x = 128
 falls inside the x layer, so the recomputed Hy right of a strip takes its
-CPML term), without CPML, FP64 and FP32 with FP16 Cb; tiles of 1 to 32
+CPML term), without CPML, FP64 and FP32 with FP32 or FP16 Cb; tiles of 1 to 32
 rows; odd run lengths (the first step in place) and runs split into
 pieces; a plane-wave line source and a spectral line monitor.
 """
@@ -103,7 +103,22 @@ def test_tm2d_equals_unfused(mi355x, shape, t, rows):
             assert rel_l2(mb.get_frequency_data(c, f), ma.get_frequency_data(c, f)) <= 1e-13
 
 
+@pytest.mark.parametrize("storage", ["auto", "fp16"])
+@pytest.mark.parametrize("shape,t", [((100, 60), 20), ((90, 70), 0)])
+def test_tm2d_fp32_equals_unfused(mi355x, shape, t, storage):
+    """FP32 fields (FP32 or FP16 Cb): the one-pass kernel against the
+    unfused pair; the same arithmetic per cell, but another instruction
+    stream, whose multiply-adds the compiler may fuse differently."""
+    a, _, _ = _run(shape, t, np.float32, storage, [131], fused=False)
+    b, _, _ = _run(shape, t, np.float32, storage, [131], fused=True, rows=3)
+    fa, fb = _fields(a), _fields(b)
+    for c in fa:
+        err = rel_l2(fb[c], fa[c])
+        assert err <= 1e-6, (c, err)
+
+
 @pytest.mark.parametrize("dtype,storage,bar", [(np.float64, "auto", 1e-10),
+                                               (np.float32, "auto", 1e-5),
                                                (np.float32, "fp16", 1e-5)])
 @pytest.mark.parametrize("shape,t,splits", [((100, 60), 20, [151]),
                                             ((150, 40), 8, [40, 33, 78]),
